@@ -1,0 +1,138 @@
+/*
+ * mbg.h - C ABI of the MI355X (gfx950) MaBoSS-semantics trajectory library, libmbg.so.
+ *
+ * Plain C types only (no C++ or torch types cross it).  Every function returns an
+ * mbg_status; errors never exit the process (the reference calls exit() from
+ * CU_CHECK/CUDA_CHECK, src/utils.cpp:3-29).  A context is not thread-safe; use one
+ * per host thread (as the reference, which runs on one host thread).  Device memory
+ * is owned by the library; host output buffers are owned by the caller.
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef MBG_H
+#define MBG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int mbg_status;
+#define MBG_OK 0
+#define MBG_ERR_PARSE 1      /* .bnd/.cfg syntax or semantic error */
+#define MBG_ERR_LIMIT 2      /* model exceeds a limit (e.g. > 20 non-internal nodes) */
+#define MBG_ERR_ARG 3        /* invalid argument or call order */
+#define MBG_ERR_HIP 4        /* HIP runtime error (no device, out of memory, launch failure) */
+#define MBG_ERR_COMPILE 5    /* hipRTC compilation failed */
+#define MBG_ERR_NOT_BUILT 6  /* no result available yet */
+
+typedef struct mbg_model mbg_model;
+typedef struct mbg_ctx mbg_ctx;
+
+/* ---------------------------------------------------------------- model (host only) */
+
+/* Parses a .bnd text then a .cfg text.  Replaces driver::parse (src/parser/driver.cpp:16-31)
+ * and the Flex/Bison front end (src/parser/scanner.ll, src/parser/parser.yy).  On failure
+ * *out is NULL and the message is copied to err. */
+mbg_status mbg_model_parse(const char* bnd_text, const char* cfg_text, mbg_model** out, char* err,
+                           size_t errlen);
+/* Same, reading the two files. */
+mbg_status mbg_model_parse_files(const char* bnd_path, const char* cfg_path, mbg_model** out, char* err,
+                                 size_t errlen);
+void mbg_model_free(mbg_model* m);
+
+/* Model facts (driver::nodes / constants, src/parser/driver.h:41-43). */
+int mbg_model_num_nodes(const mbg_model* m);
+const char* mbg_model_node_name(const mbg_model* m, int i);
+float mbg_model_node_istate(const mbg_model* m, int i);
+int mbg_model_node_is_internal(const mbg_model* m, int i); /* node_t::is_internal, parse_types.cpp:20-22 */
+int mbg_model_node_is_generic(const mbg_model* m, int i);  /* 0: "@logic ? U : 0" fast form */
+int mbg_model_num_readouts(const mbg_model* m);            /* non-internal nodes */
+/* A run constant (discrete_time, sample_count, max_time, time_tick, seed_pseudorandom) or
+ * any other constant of the .cfg.  MBG_ERR_ARG if absent. */
+mbg_status mbg_model_constant(const mbg_model* m, const char* name, double* out);
+
+/* Per-node rate_up / rate_down multipliers and istate override for the next runs of this
+ * model (variants: rate perturbations, knockouts = istate 0 and rate_up 0).  Any pointer may
+ * be NULL to restore the model's own value.  No reference counterpart (README.md:33 lists
+ * mutant analysis as future work). */
+mbg_status mbg_model_set_variant(mbg_model* m, const float* mult_up, const float* mult_down,
+                                 const float* istate);
+
+/* The hipRTC source generated for this model (replaces generator::generate_code,
+ * src/generator.cpp:11-55).  Writes at most cap bytes including the NUL; *len gets the full
+ * length.  trace != 0 adds per-trajectory summaries. */
+mbg_status mbg_model_generate_source(const mbg_model* m, int trace, char* buf, size_t cap, size_t* len);
+
+/* ---------------------------------------------------------------- device context */
+
+/* Creates a context on a HIP device (replaces kernel_compiler::kernel_compiler, which
+ * creates a CUDA context on device 0, src/kernel_compiler.cpp:46-52). */
+mbg_status mbg_create(int device, mbg_ctx** out);
+void mbg_destroy(mbg_ctx* c);
+const char* mbg_last_error(const mbg_ctx* c);
+
+/* JIT-compiles (hipRTC, gfx950) and loads the trajectory kernel for this model's shape;
+ * cached per shape.  Replaces kernel_compiler::compile_simulation
+ * (src/kernel_compiler.cpp:61-164). */
+mbg_status mbg_compile(mbg_ctx* c, const mbg_model* m, int trace);
+
+typedef struct mbg_run_params {
+    uint64_t n_trajectories; /* normalisation count n; 0 = sample_count */
+    uint64_t first_id;       /* simulate trajectory ids [first_id, first_id + count) */
+    uint64_t count;          /* 0 = n_trajectories */
+    uint64_t seed;           /* Philox key */
+    int use_model_seed;      /* 1: take seed from seed_pseudorandom */
+    uint32_t variant_id;     /* Philox counter word 3 (distinguishes variants) */
+    int trace;               /* 1: record per-trajectory summaries (count entries) */
+    int block;               /* threads per block (multiple of 64); 0 = default */
+    int grid;                /* blocks; 0 = fill the device */
+} mbg_run_params;
+
+/* Simulates and accumulates the statistics (replaces simulation_runner::run_simulation,
+ * src/simulation_runner.cu:54-180, with stats_composite::process_batch/finalize,
+ * src/statistics/stats_composite.cpp:5-20).  Synchronous. */
+mbg_status mbg_run(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p);
+
+/* Shape of the last run's tables: W windows, K readouts (2^K projected states), fixed-point
+ * exponents (cell = sum rint(t * 2^fp), th = sum rint(TH * t * 2^ft); discrete: cell is a
+ * count), number of trajectories n, transitions taken, kernel time in ms. */
+typedef struct mbg_run_info {
+    int windows, readouts, prob_exponent, th_exponent, rate_exponent, state_words;
+    int discrete;
+    uint64_t n_trajectories, simulated, transitions, fixed_point_trajectories;
+    double kernel_ms;
+    float time_tick, max_time;
+} mbg_run_info;
+mbg_status mbg_get_run_info(const mbg_ctx* c, mbg_run_info* out);
+
+/* Window tables (window_average_small_stats::finalize, src/statistics/window_average_small.cpp:
+ * 109-140): prob[W * 2^K] and th[W] exact fixed-point sums; either pointer may be NULL. */
+mbg_status mbg_get_probtraj_raw(const mbg_ctx* c, uint64_t* prob, uint64_t* th);
+/* Normalised: P = cell / (n * tick) (or count / n), TH likewise. */
+mbg_status mbg_get_probtraj(const mbg_ctx* c, double* prob, double* th);
+/* Final projected states, counts[2^K] (final_states_stats, src/statistics/final_states.cpp:45-66). */
+mbg_status mbg_get_final_states(const mbg_ctx* c, uint64_t* counts);
+/* Distinct fixed points in the reference order (most significant word first,
+ * src/state.h:43-49; fixed_states_stats, src/statistics/fixed_states.cu:84-187): states
+ * [cap][state_words] and counts [cap]; *n gets the number of distinct fixed points. */
+mbg_status mbg_get_fixed_points(const mbg_ctx* c, uint64_t* states, uint64_t* counts, size_t cap, size_t* n);
+/* Per-trajectory summaries of a trace run: [count][4] = flips, FNV-1a hash of the flipped
+ * node indices, final time (FP64 bits), status (1 finished, 2 fixed point). */
+mbg_status mbg_get_summaries(const mbg_ctx* c, uint64_t* out, size_t count);
+
+/* Output files of the reference: <prefix>_probtraj.csv (window_average_small_stats::write_csv,
+ * src/statistics/window_average_small.cpp:181-250) and <prefix>_fp.csv
+ * (fixed_states_stats::write_csv, src/statistics/fixed_states.cu:154-187). */
+mbg_status mbg_write_csv(const mbg_ctx* c, const mbg_model* m, const char* prefix);
+/* Stdout format (stats_composite::visualize: final_states.cpp:53-66, fixed_states.cu:141-152,
+ * window_average_small.cpp:142-179) written to path ("-" = stdout). */
+mbg_status mbg_write_text(const mbg_ctx* c, const mbg_model* m, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MBG_H */

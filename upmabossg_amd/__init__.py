@@ -1,0 +1,299 @@
+"""MI355X-native MaBoSS-semantics stochastic Boolean network simulator.
+
+Python binding of libmbg.so (C ABI: include/mbg.h).  The simulation runs only
+in the HIP trajectory kernel; there is no CPU fallback: if libmbg.so is not
+built, importing the binding raises.
+
+    from upmabossg_amd import Model, Context
+    m = Model.from_files("model.bnd", "model.cfg")
+    with Context() as ctx:
+        r = ctx.run(m)                 # sample_count trajectories, seed_pseudorandom
+        r.probtraj, r.final_states, r.fixed_points
+        ctx.write_csv(m, "out")         # out_probtraj.csv, out_fp.csv
+
+The same run from the shell: ``upmabossg_amd/MaBoSSG -o out model.bnd model.cfg``
+(the reference CLI, src/main.cpp:101-169).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+__all__ = ["Model", "Context", "RunResult", "MbgError", "lib", "LIB_PATH", "CLI_PATH"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmbg.so")
+CLI_PATH = os.path.join(_HERE, "MaBoSSG")
+
+MBG_OK = 0
+_STATUS = {1: "parse error", 2: "limit exceeded", 3: "invalid argument", 4: "HIP error",
+           5: "hipRTC compilation error", 6: "no result yet"}
+
+
+class MbgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class _RunParams(C.Structure):
+    _fields_ = [
+        ("n_trajectories", C.c_uint64),
+        ("first_id", C.c_uint64),
+        ("count", C.c_uint64),
+        ("seed", C.c_uint64),
+        ("use_model_seed", C.c_int),
+        ("variant_id", C.c_uint32),
+        ("trace", C.c_int),
+        ("block", C.c_int),
+        ("grid", C.c_int),
+    ]
+
+
+class _RunInfo(C.Structure):
+    _fields_ = [
+        ("windows", C.c_int), ("readouts", C.c_int), ("prob_exponent", C.c_int), ("th_exponent", C.c_int),
+        ("rate_exponent", C.c_int), ("state_words", C.c_int), ("discrete", C.c_int),
+        ("n_trajectories", C.c_uint64), ("simulated", C.c_uint64), ("transitions", C.c_uint64),
+        ("fixed_point_trajectories", C.c_uint64), ("kernel_ms", C.c_double),
+        ("time_tick", C.c_float), ("max_time", C.c_float),
+    ]
+
+
+_LIB = None
+
+
+def lib():
+    """Loads libmbg.so; raises if it is not built (no fallback path exists)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run `python -m upmabossg_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, cp, sz = C.c_void_p, C.c_char_p, C.c_size_t
+    u64p, f64p, f32p = C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_float)
+    sig = {
+        "mbg_model_parse": (C.c_int, [cp, cp, C.POINTER(vp), cp, sz]),
+        "mbg_model_parse_files": (C.c_int, [cp, cp, C.POINTER(vp), cp, sz]),
+        "mbg_model_free": (None, [vp]),
+        "mbg_model_num_nodes": (C.c_int, [vp]),
+        "mbg_model_node_name": (cp, [vp, C.c_int]),
+        "mbg_model_node_istate": (C.c_float, [vp, C.c_int]),
+        "mbg_model_node_is_internal": (C.c_int, [vp, C.c_int]),
+        "mbg_model_node_is_generic": (C.c_int, [vp, C.c_int]),
+        "mbg_model_num_readouts": (C.c_int, [vp]),
+        "mbg_model_constant": (C.c_int, [vp, cp, f64p]),
+        "mbg_model_set_variant": (C.c_int, [vp, f32p, f32p, f32p]),
+        "mbg_model_generate_source": (C.c_int, [vp, C.c_int, cp, sz, C.POINTER(sz)]),
+        "mbg_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "mbg_destroy": (None, [vp]),
+        "mbg_last_error": (cp, [vp]),
+        "mbg_compile": (C.c_int, [vp, vp, C.c_int]),
+        "mbg_run": (C.c_int, [vp, vp, C.POINTER(_RunParams)]),
+        "mbg_get_run_info": (C.c_int, [vp, C.POINTER(_RunInfo)]),
+        "mbg_get_probtraj_raw": (C.c_int, [vp, u64p, u64p]),
+        "mbg_get_probtraj": (C.c_int, [vp, f64p, f64p]),
+        "mbg_get_final_states": (C.c_int, [vp, u64p]),
+        "mbg_get_fixed_points": (C.c_int, [vp, u64p, u64p, sz, C.POINTER(sz)]),
+        "mbg_get_summaries": (C.c_int, [vp, u64p, sz]),
+        "mbg_write_csv": (C.c_int, [vp, vp, cp]),
+        "mbg_write_text": (C.c_int, [vp, vp, cp]),
+    }
+    for name, (rt, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = rt
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def _u64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+class Model:
+    """A parsed and compiled .bnd/.cfg model (host only; no GPU needed)."""
+
+    def __init__(self, bnd_text: str, cfg_text: str):
+        L = lib()
+        h = C.c_void_p()
+        err = C.create_string_buffer(4096)
+        st = L.mbg_model_parse(bnd_text.encode(), cfg_text.encode(), C.byref(h), err, 4096)
+        if st != MBG_OK:
+            raise MbgError(st, err.value.decode())
+        self._h = h
+
+    @classmethod
+    def from_files(cls, bnd_path: str, cfg_path: str) -> "Model":
+        with open(bnd_path) as f:
+            b = f.read()
+        with open(cfg_path) as f:
+            c = f.read()
+        return cls(b, c)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _LIB is not None:
+            _LIB.mbg_model_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def num_nodes(self) -> int:
+        return lib().mbg_model_num_nodes(self._h)
+
+    @property
+    def num_readouts(self) -> int:
+        return lib().mbg_model_num_readouts(self._h)
+
+    def names(self):
+        return [lib().mbg_model_node_name(self._h, i).decode() for i in range(self.num_nodes)]
+
+    def istate(self):
+        return np.array([lib().mbg_model_node_istate(self._h, i) for i in range(self.num_nodes)], np.float32)
+
+    def internal(self):
+        return np.array([lib().mbg_model_node_is_internal(self._h, i) for i in range(self.num_nodes)], bool)
+
+    def generic(self):
+        return np.array([lib().mbg_model_node_is_generic(self._h, i) for i in range(self.num_nodes)], bool)
+
+    def constant(self, name: str) -> float:
+        v = C.c_double()
+        st = lib().mbg_model_constant(self._h, name.encode(), C.byref(v))
+        if st != MBG_OK:
+            raise KeyError(name)
+        return v.value
+
+    def set_variant(self, mult_up=None, mult_down=None, istate=None):
+        keep = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            if a.shape != (self.num_nodes,):
+                raise ValueError("per-node arrays must have one entry per node")
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_float))
+
+        st = lib().mbg_model_set_variant(self._h, ptr(mult_up), ptr(mult_down), ptr(istate))
+        if st != MBG_OK:
+            raise MbgError(st, "invalid variant parameters")
+
+    def generate_source(self, trace=False) -> str:
+        n = C.c_size_t()
+        lib().mbg_model_generate_source(self._h, int(trace), None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value + 1)
+        lib().mbg_model_generate_source(self._h, int(trace), buf, n.value + 1, C.byref(n))
+        return buf.value.decode()
+
+
+@dataclass
+class RunResult:
+    windows: int
+    readouts: int
+    prob_exponent: int
+    th_exponent: int
+    rate_exponent: int
+    discrete: bool
+    n_trajectories: int
+    simulated: int
+    transitions: int
+    fixed_point_trajectories: int
+    kernel_ms: float
+    time_tick: float
+    max_time: float
+    prob_raw: np.ndarray       # (W, 2^K) uint64 fixed point (counts when discrete)
+    th_raw: np.ndarray         # (W,) uint64 fixed point
+    probtraj: np.ndarray       # (W, 2^K) float64 normalised
+    th: np.ndarray             # (W,) float64
+    final_states: np.ndarray   # (2^K,) uint64 counts
+    fixed_states: np.ndarray   # (F, words) uint64, reference order
+    fixed_counts: np.ndarray   # (F,) uint64
+    summaries: np.ndarray | None  # (count, 4) uint64 when traced
+
+
+class Context:
+    """A HIP device context (one per host thread)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        st = L.mbg_create(device, C.byref(h))
+        if st != MBG_OK:
+            raise MbgError(st, f"cannot create a context on HIP device {device}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.mbg_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st):
+        if st != MBG_OK:
+            raise MbgError(st, lib().mbg_last_error(self._h).decode())
+
+    def compile(self, model: Model, trace=False):
+        self._check(lib().mbg_compile(self._h, model.handle, int(trace)))
+
+    def run(self, model: Model, n_trajectories=0, count=0, first_id=0, seed=None, variant_id=0, trace=False,
+            block=0, grid=0, fetch=True) -> RunResult | None:
+        p = _RunParams(n_trajectories, first_id, count, 0 if seed is None else seed, int(seed is None),
+                       variant_id, int(trace), block, grid)
+        self._check(lib().mbg_run(self._h, model.handle, C.byref(p)))
+        return self.result(trace) if fetch else None
+
+    def info(self) -> _RunInfo:
+        i = _RunInfo()
+        self._check(lib().mbg_get_run_info(self._h, C.byref(i)))
+        return i
+
+    def result(self, with_summaries=False) -> RunResult:
+        L = lib()
+        i = self.info()
+        W, S = i.windows, 1 << i.readouts
+        prob_raw = np.zeros(W * S, np.uint64)
+        th_raw = np.zeros(W, np.uint64)
+        self._check(L.mbg_get_probtraj_raw(self._h, _u64p(prob_raw), _u64p(th_raw)))
+        prob = np.zeros(W * S, np.float64)
+        th = np.zeros(W, np.float64)
+        self._check(L.mbg_get_probtraj(self._h, prob.ctypes.data_as(C.POINTER(C.c_double)),
+                                       th.ctypes.data_as(C.POINTER(C.c_double))))
+        fin = np.zeros(S, np.uint64)
+        self._check(L.mbg_get_final_states(self._h, _u64p(fin)))
+        n = C.c_size_t()
+        self._check(L.mbg_get_fixed_points(self._h, None, None, 0, C.byref(n)))
+        fs = np.zeros((n.value, i.state_words), np.uint64)
+        fc = np.zeros(n.value, np.uint64)
+        if n.value:
+            self._check(L.mbg_get_fixed_points(self._h, _u64p(fs), _u64p(fc), n.value, C.byref(n)))
+        summ = None
+        if with_summaries:
+            summ = np.zeros((i.simulated, 4), np.uint64)
+            self._check(L.mbg_get_summaries(self._h, _u64p(summ), i.simulated))
+        return RunResult(W, i.readouts, i.prob_exponent, i.th_exponent, i.rate_exponent, bool(i.discrete),
+                         i.n_trajectories, i.simulated, i.transitions, i.fixed_point_trajectories, i.kernel_ms,
+                         i.time_tick, i.max_time, prob_raw.reshape(W, S), th_raw, prob.reshape(W, S), th, fin,
+                         fs, fc, summ)
+
+    def write_csv(self, model: Model, prefix: str):
+        self._check(lib().mbg_write_csv(self._h, model.handle, prefix.encode()))
+
+    def write_text(self, model: Model, path: str = "-"):
+        self._check(lib().mbg_write_text(self._h, model.handle, path.encode()))

@@ -1,0 +1,302 @@
+#include "model.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <set>
+
+#define MBG_HD static inline
+#include "device/mbg_math.h"
+
+namespace mbg {
+
+namespace {
+
+// The reference's fast-path test (src/generator.cpp:57-82): "@logic ? U : 0" for rate_up and
+// "@logic ? 0 : D" for rate_down, with U and D constant expressions.
+bool fast_form(const ParsedModel& pm, const Ast* e, bool up, float& val) {
+    if (!e || e->kind != Ast::Ternary) return false;
+    const Ast* c = e->kids[0].get();
+    if (c->kind != Ast::Alias || c->text != "@logic") return false;
+    float l, r;
+    try {
+        l = evaluate(e->kids[1].get(), pm);
+        r = evaluate(e->kids[2].get(), pm);
+    } catch (const ParseError&) {
+        return false;
+    }
+    val = up ? l : r;
+    return (up ? r : l) == 0.f;
+}
+
+struct StateEval {
+    const ParsedModel& pm;
+    int self;
+    const std::vector<uint8_t>& bits;  // per node 0/1
+    int depth = 0;
+
+    float eval(const Ast* e) {
+        switch (e->kind) {
+            case Ast::Number: return e->number;
+            case Ast::Variable: return evaluate(e, pm);
+            case Ast::Group: return eval(e->kids[0].get());
+            case Ast::Node: {
+                const int i = pm.index_of(e->text);
+                if (i < 0) throw ParseError("unknown node name: " + e->text);
+                return bits[i] ? 1.f : 0.f;
+            }
+            case Ast::Alias: {
+                if (++depth > 64) throw ParseError("alias recursion too deep");
+                const Ast* a = pm.nodes[self].attr(e->text.substr(1));
+                if (!a) throw ParseError("Node " + pm.nodes[self].name + " does not have attribute " + e->text.substr(1));
+                const float v = eval(a);
+                depth--;
+                return v;
+            }
+            case Ast::Unary: {
+                const float v = eval(e->kids[0].get());
+                return e->oper == Ast::Sub ? -v : e->oper == Ast::Not ? (v == 0.f ? 1.f : 0.f) : v;
+            }
+            case Ast::Ternary: {
+                const float c = eval(e->kids[0].get());
+                const float m = eval(e->kids[1].get());
+                const float r = eval(e->kids[2].get());
+                return c != 0.f ? m : r;
+            }
+            case Ast::Binary: {
+                const float l = eval(e->kids[0].get());
+                const float r = eval(e->kids[1].get());
+                switch (e->oper) {
+                    case Ast::Add: return l + r;
+                    case Ast::Sub: return l - r;
+                    case Ast::Mul: return l * r;
+                    case Ast::Div: return l / r;
+                    case Ast::And: return (l != 0.f && r != 0.f) ? 1.f : 0.f;
+                    case Ast::Or: return (l != 0.f || r != 0.f) ? 1.f : 0.f;
+                    case Ast::Xor: return ((l != 0.f) != (r != 0.f)) ? 1.f : 0.f;
+                    case Ast::Eq: return l == r ? 1.f : 0.f;
+                    case Ast::Ne: return l != r ? 1.f : 0.f;
+                    case Ast::Lt: return l < r ? 1.f : 0.f;
+                    case Ast::Le: return l <= r ? 1.f : 0.f;
+                    case Ast::Gt: return l > r ? 1.f : 0.f;
+                    case Ast::Ge: return l >= r ? 1.f : 0.f;
+                    default: break;
+                }
+            }
+        }
+        throw ParseError("malformed expression");
+    }
+};
+
+void collect_nodes(const ParsedModel& pm, int self, const Ast* e, std::set<int>& out, int depth = 0) {
+    if (depth > 64) throw ParseError("alias recursion too deep");
+    switch (e->kind) {
+        case Ast::Node: {
+            const int i = pm.index_of(e->text);
+            if (i < 0) throw ParseError("unknown node name: " + e->text);
+            out.insert(i);
+            return;
+        }
+        case Ast::Alias: {
+            const Ast* a = pm.nodes[self].attr(e->text.substr(1));
+            if (!a) throw ParseError("Node " + pm.nodes[self].name + " does not have attribute " + e->text.substr(1));
+            collect_nodes(pm, self, a, out, depth + 1);
+            return;
+        }
+        default:
+            for (const auto& k : e->kids) collect_nodes(pm, self, k.get(), out, depth);
+    }
+}
+
+void check_rate(float v, const std::string& node) {
+    if (!(v >= 0.f) || std::isinf(v))
+        throw ParseError("node " + node + ": rates must be finite and non-negative");
+}
+
+}  // namespace
+
+int ceil_log2_u64(uint64_t v) {
+    int e = 0;
+    while (e < 64 && (1ull << e) < v) e++;
+    return e;
+}
+
+int ceil_log2_real(double q) {
+    int e = 0;
+    while (std::ldexp(1.0, e) < q) e++;
+    while (e > -1000 && std::ldexp(1.0, e - 1) >= q) e--;
+    return e;
+}
+
+CompiledModel compile_model(const ParsedModel& pm) {
+    CompiledModel cm;
+    cm.N = (int)pm.nodes.size();
+    if (cm.N == 0) throw ParseError("model has no nodes");
+    if (cm.N >= (1 << 24)) throw ParseError("too many nodes");
+    cm.NW = (cm.N + 63) / 64;
+    cm.discrete = pm.constant("discrete_time") != 0.f;
+    cm.max_time = pm.constant("max_time");
+    cm.time_tick = pm.constant("time_tick");
+    cm.sample_count = pm.constant("sample_count");
+    cm.seed = pm.constant("seed_pseudorandom");
+    cm.nodes.resize(cm.N);
+    cm.generic.assign(cm.N, 0);
+    cm.internal.assign(cm.N, 0);
+    cm.up.assign(cm.N, 0.f);
+    cm.down.assign(cm.N, 0.f);
+    cm.gen_mask.assign(cm.NW, 0);
+    for (int i = 0; i < cm.N; i++) {
+        const NodeDecl& nd = pm.nodes[i];
+        cm.names.push_back(nd.name);
+        cm.istate.push_back(nd.istate);
+        const Ast* ii = nd.attr("is_internal");  // src/parser/parse_types.cpp:20-22
+        cm.internal[i] = ii && evaluate(ii, pm) != 0.f;
+        if (!cm.internal[i]) cm.readouts.push_back(i);
+    }
+    cm.K = (int)cm.readouts.size();
+    for (int j = 0; j < cm.K; j++) cm.nodes[cm.readouts[j]].flags = (uint8_t)(j + 1);
+
+    std::vector<std::vector<uint32_t>> dep_lists(cm.N);
+    std::vector<uint8_t> bits(cm.N, 0);
+    uint64_t lut_bits = 0;
+    for (int i = 0; i < cm.N; i++) {
+        const NodeDecl& nd = pm.nodes[i];
+        NodeRec& rec = cm.nodes[i];
+        float u = 0.f, d = 0.f;
+        const bool fast = fast_form(pm, nd.attr("rate_up"), true, u) && fast_form(pm, nd.attr("rate_down"), false, d);
+        std::set<int> refs;
+        const Ast* logic = nd.attr("logic");
+        if (fast) {
+            if (!logic) throw ParseError("Node " + nd.name + " does not have attribute logic");
+            collect_nodes(pm, i, logic, refs);
+        } else {
+            collect_nodes(pm, i, nd.attr("rate_up"), refs);
+            collect_nodes(pm, i, nd.attr("rate_down"), refs);
+        }
+        refs.erase(i);
+        const int cnt = (int)refs.size();
+        if (cnt > (fast ? kMaxInputsFast : kMaxInputsGeneric))
+            throw ParseError("node " + nd.name + " reads " + std::to_string(cnt) + " nodes (limit " +
+                             std::to_string(fast ? kMaxInputsFast : kMaxInputsGeneric) + ")");
+        rec.in_off = (uint32_t)cm.inputs.size();
+        rec.in_cnt = (uint8_t)cnt;
+        std::vector<int> in(refs.begin(), refs.end());
+        for (int m = 0; m < cnt; m++) {
+            cm.inputs.push_back((uint32_t)in[m]);
+            dep_lists[in[m]].push_back((uint32_t)i | ((uint32_t)(m + 1) << 24));
+        }
+        dep_lists[i].push_back((uint32_t)i);  // every rate reads its own node (key bit 0)
+        const uint32_t nkeys = 1u << (cnt + 1);
+        StateEval ev{pm, i, bits};
+        if (fast) {
+            check_rate(u, nd.name);
+            check_rate(d, nd.name);
+            cm.up[i] = u;
+            cm.down[i] = d;
+            rec.tab_off = (uint32_t)lut_bits;
+            cm.lut.resize((lut_bits + nkeys + 31) / 32, 0u);
+            for (uint32_t key = 0; key < nkeys; key++) {
+                bits[i] = key & 1u;
+                for (int m = 0; m < cnt; m++) bits[in[m]] = (key >> (m + 1)) & 1u;
+                if (ev.eval(logic) != 0.f) cm.lut[(lut_bits + key) >> 5] |= 1u << ((lut_bits + key) & 31);
+            }
+            lut_bits += nkeys;
+        } else {
+            cm.generic[i] = 1;
+            rec.flags |= 0x80;
+            cm.gen_mask[i >> 6] |= 1ull << (i & 63);
+            rec.tab_off = (uint32_t)cm.gtab.size();
+            const Ast* ru = nd.attr("rate_up");
+            const Ast* rd = nd.attr("rate_down");
+            for (uint32_t key = 0; key < nkeys; key++) {
+                bits[i] = key & 1u;
+                for (int m = 0; m < cnt; m++) bits[in[m]] = (key >> (m + 1)) & 1u;
+                const float v = (key & 1u) ? ev.eval(rd) : ev.eval(ru);
+                check_rate(v, nd.name);
+                cm.gtab.push_back(v);
+            }
+        }
+        bits[i] = 0;
+        for (int m = 0; m < cnt; m++) bits[in[m]] = 0;
+    }
+    for (int i = 0; i < cm.N; i++) {
+        if (dep_lists[i].size() > 0xffff) throw ParseError("node " + cm.names[i] + " has too many dependents");
+        cm.nodes[i].dep_off = (uint32_t)cm.deps.size();
+        cm.nodes[i].dep_cnt = (uint16_t)dep_lists[i].size();
+        cm.deps.insert(cm.deps.end(), dep_lists[i].begin(), dep_lists[i].end());
+    }
+    return cm;
+}
+
+RateTables make_rate_tables(const CompiledModel& cm, const float* mult_up, const float* mult_down,
+                            const float* istate) {
+    RateTables t;
+    std::vector<float> up(cm.N), down(cm.N), gt = cm.gtab;
+    for (int i = 0; i < cm.N; i++) {
+        const float fu = mult_up ? mult_up[i] : 1.f, fd = mult_down ? mult_down[i] : 1.f;
+        if (!(fu >= 0.f) || !(fd >= 0.f) || std::isinf(fu) || std::isinf(fd))
+            throw ParseError("rate multipliers must be finite and non-negative");
+        if (cm.generic[i]) {
+            const NodeRec& r = cm.nodes[i];
+            const uint32_t nkeys = 1u << (r.in_cnt + 1);
+            for (uint32_t key = 0; key < nkeys; key++) gt[r.tab_off + key] = cm.gtab[r.tab_off + key] * ((key & 1u) ? fd : fu);
+        } else {
+            up[i] = cm.up[i] * fu;
+            down[i] = cm.down[i] * fd;
+        }
+    }
+    // Exponent: every non-zero rate an integer multiple of 2^-E where possible,
+    // capped so that N * max_rate * 2^E < 2^62 (exact 64-bit sums).
+    int e_exact = INT_MIN;
+    float vmax = 0.f;
+    auto visit = [&](float v) {
+        if (v == 0.f) return;
+        check_rate(v, "variant");
+        int ex;
+        const double fr = std::frexp((double)v, &ex);
+        const uint32_t mant = (uint32_t)std::ldexp(fr, 24);
+        e_exact = std::max(e_exact, -(ex - 24 + __builtin_ctz(mant)));
+        vmax = std::max(vmax, v);
+    };
+    for (int i = 0; i < cm.N; i++)
+        if (!cm.generic[i]) {
+            visit(up[i]);
+            visit(down[i]);
+        }
+    for (float v : gt) visit(v);
+    if (vmax == 0.f) {
+        t.E = 0;
+    } else {
+        int ev;
+        std::frexp((double)vmax, &ev);
+        t.E = std::min(e_exact, 62 - ev - ceil_log2_u64((uint64_t)cm.N));
+    }
+    auto fixed = [&](float v) -> uint64_t {
+        if (v == 0.f) return 0;
+        double r = std::nearbyint(std::ldexp((double)v, t.E));
+        return r < 1.0 ? 1ull : (uint64_t)r;
+    };
+    t.rates.resize(2 * (size_t)cm.N);
+    for (int i = 0; i < cm.N; i++) {
+        t.rates[2 * i] = fixed(up[i]);
+        t.rates[2 * i + 1] = fixed(down[i]);
+    }
+    t.rtab.resize(gt.size());
+    for (size_t k = 0; k < gt.size(); k++) t.rtab[k] = fixed(gt[k]);
+    t.ro_lg.assign(2 * (size_t)cm.K, 0.0);
+    for (int j = 0; j < cm.K; j++) {
+        const int n = cm.readouts[j];
+        for (int b = 0; b < 2; b++) {
+            const uint64_t r = t.rates[2 * n + b];
+            if (r && !cm.generic[n]) {
+                const double rd = (double)r;
+                t.ro_lg[2 * j + b] = rd * mbg_log2(rd);
+            }
+        }
+    }
+    t.istate.resize(cm.N);
+    for (int i = 0; i < cm.N; i++) t.istate[i] = istate ? istate[i] : cm.istate[i];
+    return t;
+}
+
+}  // namespace mbg

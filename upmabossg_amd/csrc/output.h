@@ -1,0 +1,36 @@
+// Output formats of the reference: <prefix>_probtraj.csv, <prefix>_fp.csv and the stdout dump.
+#pragma once
+
+#include <cstdint>
+#include <ostream>
+#include <string>
+#include <vector>
+
+#include "../../include/mbg.h"
+#include "model.h"
+
+namespace mbg {
+
+struct Results {
+    const mbg_run_info* info;
+    const std::vector<uint64_t>* prob;          // [W][2^K] fixed point (counts when discrete)
+    const std::vector<uint64_t>* th;            // [W]
+    const std::vector<uint64_t>* fin;           // [2^K]
+    const std::vector<uint64_t>* fixed_states;  // [F][state_words], reference order
+    const std::vector<uint64_t>* fixed_counts;  // [F]
+};
+
+// Normalised probability of cell (window-major) and window transition entropy
+// (src/statistics/window_average_small.cpp:126-140, :222-223).
+double window_prob(const Results& r, size_t cell);
+double window_th(const Results& r, size_t w);
+
+// " -- "-joined names of the ON nodes, or "<nil>" (src/state.cpp:29-48).
+std::string state_string(const CompiledModel& cm, const uint64_t* words);
+std::string projected_string(const CompiledModel& cm, uint64_t s);
+
+bool write_probtraj_csv(const Results& r, const CompiledModel& cm, const std::string& path);
+bool write_fp_csv(const Results& r, const CompiledModel& cm, const std::string& path);
+void write_text(const Results& r, const CompiledModel& cm, std::ostream& os);
+
+}  // namespace mbg
