@@ -1,0 +1,162 @@
+"""GPU trajectory kernel vs the CPU oracle, through the C ABI (libmbg.so).
+
+Bit-exact: given the same Philox draws, each trajectory's transition count, FNV-1a hash of
+the flipped node sequence, final time and status match the oracle's replay, and the window
+tables, final-state counts and fixed-point multisets are identical integers.  At full sizes
+(1,048,576 trajectories of the 1,000-node network) the checks are size-independent properties:
+windows sum to one, counts add up, results do not depend on block size or grid.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleModel
+from tests.conftest import REFERENCE_MODELS, model_text
+from upmabossg_amd import Model, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _compare(ctx, bnd, cfg, count, n=None, first_id=0, seed=None, threads=16, summaries=True, block=0):
+    n = n or count
+    m = Model(bnd, cfg)
+    o = OracleModel(bnd, cfg)
+    _flush(f"case: N={m.num_nodes} count={count} first_id={first_id} block={block}")
+    g = ctx.run(m, n_trajectories=n, count=count, first_id=first_id, seed=seed, trace=summaries, block=block)
+    r = o.run(count=count, first_id=first_id, n_trajectories=n, seed=seed, threads=threads,
+              record_summaries=summaries)
+    if summaries:
+        s = r.summaries
+        assert np.array_equal(g.summaries[:, 0], s["flips"]), "transition counts differ"
+        assert np.array_equal(g.summaries[:, 1], s["flip_hash"]), "flip sequences differ"
+        assert np.array_equal(g.summaries[:, 2], s["final_time"].view(np.uint64)), "final times differ"
+        assert np.array_equal(g.summaries[:, 3], s["status"].astype(np.uint64)), "statuses differ"
+    assert g.transitions == r.flips
+    assert g.prob_exponent == r.prob_exponent and g.th_exponent == r.th_exponent
+    assert np.array_equal(g.prob_raw, r.prob), "window tables differ"
+    assert np.array_equal(g.th_raw, r.th), "window entropies differ"
+    assert np.array_equal(g.final_states, r.final), "final states differ"
+    assert np.array_equal(g.fixed_states, r.fixed_states), "fixed points differ"
+    assert np.array_equal(g.fixed_counts, r.fixed_counts), "fixed-point counts differ"
+    return g, r
+
+
+def test_small_network_bit_exact(gpu_ctx):
+    net = synth.config_small()
+    g, r = _compare(gpu_ctx, net.bnd(), net.cfg(), count=10000)
+    assert g.transitions > 100000
+
+
+@pytest.mark.parametrize("name,count", [("cellcycle", 20000), ("metastasis", 1000), ("sizek", 2000),
+                                        ("Montagud2022_Prostate_Cancer", 2000)])
+def test_reference_models_bit_exact(gpu_ctx, name, count):
+    b, c = model_text(name)
+    _compare(gpu_ctx, b, c, count=count)
+
+
+def test_large_network_bit_exact(gpu_ctx):
+    net = config = synth.config_large()
+    _compare(gpu_ctx, net.bnd(), net.cfg(), count=2048, n=1048576)
+    # a slice from the middle of the id range
+    _compare(gpu_ctx, config.bnd(), config.cfg(), count=512, n=1048576, first_id=700000)
+
+
+def test_fixed_point_heavy_bit_exact(gpu_ctx):
+    net = synth.config_fixed_heavy()
+    g, r = _compare(gpu_ctx, net.bnd(), net.cfg(), count=256, n=262144)
+
+
+def test_block_size_and_grid_independent(gpu_ctx):
+    net = synth.config_small()
+    m = Model(net.bnd(), net.cfg())
+    a = gpu_ctx.run(m, count=5000, trace=True, block=64)
+    b = gpu_ctx.run(m, count=5000, trace=True, block=256, grid=3)
+    assert np.array_equal(a.summaries, b.summaries)
+    assert np.array_equal(a.prob_raw, b.prob_raw) and np.array_equal(a.th_raw, b.th_raw)
+
+
+def test_variant_multipliers_and_knockout(gpu_ctx):
+    net = synth.config_large()
+    rng = np.random.default_rng(3)
+    mu = np.exp(rng.normal(0, 0.5, net.n)).astype(np.float32)
+    md = np.exp(rng.normal(0, 0.5, net.n)).astype(np.float32)
+    ist = net.istate.copy()
+    for k in (5, 17, 400):  # knockouts: forced OFF
+        mu[k] = 0.0
+        ist[k] = 0.0
+    m = Model(net.bnd(), net.cfg())
+    m.set_variant(mu, md, ist)
+    o = OracleModel(net.bnd(), net.cfg())
+    o.set_variant(mu, md, ist)
+    g = gpu_ctx.run(m, count=1024, n_trajectories=1024, variant_id=7, trace=True)
+    r = o.run(count=1024, variant_id=7, threads=16, record_summaries=True)
+    assert np.array_equal(g.summaries[:, 1], r.summaries["flip_hash"])
+    assert np.array_equal(g.prob_raw, r.prob) and np.array_equal(g.final_states, r.final)
+
+
+EDGE_MODELS = {
+    # immediate fixed point: every node already agrees with its logic
+    "frozen": ("node A { logic = A; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n"
+               "node B { logic = !A; rate_up = @logic ? 2 : 0; rate_down = @logic ? 0 : 2; }\n",
+               "A.istate = 1; B.istate = 0; max_time = 5; time_tick = 1; sample_count = 100;"),
+    # generic (non fast-path) rate expressions, XOR, arithmetic, a self-loop
+    "generic": ("node A { logic = A ^ B; rate_up = B ? 2.5 * $k : 0.5; rate_down = @logic ? 0 : 1 + C; }\n"
+                "node B { logic = A | C; rate_up = @logic ? $k : 0; rate_down = @logic ? 0 : 3; }\n"
+                "node C { logic = !C; rate_up = A + B; rate_down = 1.25; }\n",
+                "$k = 0.75; C.is_internal = 1; max_time = 7.5; time_tick = 0.35; sample_count = 3000;"),
+    # no readouts at all (every node internal), max_time shorter than the tick
+    "internal_only": ("node A { logic = !B; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n"
+                      "node B { logic = A; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n",
+                      "A.is_internal = 1; B.is_internal = 1; max_time = 0.5; time_tick = 2; sample_count = 500;"),
+    # discrete time with a fixed point reached mid-run
+    "discrete_fixed": ("node A { logic = A | B; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n"
+                       "node B { logic = !A; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n",
+                       "discrete_time = 1; max_time = 3; time_tick = 0.3; sample_count = 2000;"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(EDGE_MODELS))
+def test_edge_models_bit_exact(gpu_ctx, name):
+    b, c = EDGE_MODELS[name]
+    m = Model(b, c)
+    n = int(m.constant("sample_count"))
+    _compare(gpu_ctx, b, c, count=n)
+
+
+def test_single_trajectory(gpu_ctx):
+    net = synth.config_large()
+    _compare(gpu_ctx, net.bnd(), net.cfg(), count=1, n=1)
+
+
+def test_large_full_size_properties(gpu_ctx):
+    """1,048,576 trajectories of the 1,000-node network: every window's probabilities sum to
+    one, final states count every trajectory, and two runs give identical tables."""
+    net = synth.config_large()
+    m = Model(net.bnd(), net.cfg())
+    _flush("case: large full size, run 1")
+    a = gpu_ctx.run(m)
+    assert a.simulated == 1048576
+    S = a.prob_raw.sum(axis=1)
+    full = np.float64(2.0) ** a.prob_exponent * a.n_trajectories * np.float64(np.float32(0.5))
+    assert np.all(np.abs(S.astype(np.float64) - full) <= 1e-9 * full + a.n_trajectories * 300)
+    assert int(a.final_states.sum()) == 1048576
+    assert np.all(a.th >= 0)
+    _flush("case: large full size, run 2")
+    b = gpu_ctx.run(m)
+    assert np.array_equal(a.prob_raw, b.prob_raw) and np.array_equal(a.th_raw, b.th_raw)
+    assert np.array_equal(a.final_states, b.final_states) and a.transitions == b.transitions
+    # 4-sigma agreement of the readout marginals with an oracle sample of 4,096 trajectories
+    o = OracleModel(net.bnd(), net.cfg())
+    r = o.run(count=4096, threads=16)
+    K = a.readouts
+    S_ = 1 << K
+    bits = (np.arange(S_)[:, None] >> np.arange(K)[None, :]) & 1
+    pg = a.probtraj @ bits
+    po = (r.prob.astype(np.float64) / 2.0 ** r.prob_exponent / (4096 * 0.5)) @ bits
+    sig = np.sqrt(np.maximum(pg * (1 - pg), 1e-4) / 4096)
+    assert np.all(np.abs(pg - po) <= 4.5 * sig)

@@ -1,0 +1,122 @@
+"""libmbg.so on the CPU side (no GPU calls): exported symbols, front end, errors, codegen, CLI."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleModel
+from tests.conftest import REFERENCE_MODELS, ROOT, model_text
+from upmabossg_amd import CLI_PATH, LIB_PATH, MbgError, Model, synth
+
+
+def _declared_symbols():
+    with open(os.path.join(ROOT, "include", "mbg.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(mbg_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB_PATH)
+    syms = _declared_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(lib, s), s
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (mbg_[a-z_]+)$", out, re.M))
+    assert exported == set(syms)
+
+
+@pytest.mark.parametrize("name", REFERENCE_MODELS)
+def test_front_end_matches_oracle_on_reference_models(name):
+    b, c = model_text(name)
+    m = Model(b, c)
+    o = OracleModel(b, c)
+    assert m.names() == o.names()
+    assert np.array_equal(m.istate(), o.istate())
+    assert np.array_equal(m.internal(), o.internal())
+    assert not m.generic().any()
+    for k in ["discrete_time", "sample_count", "max_time", "time_tick", "seed_pseudorandom"]:
+        assert m.constant(k) == o.constant(k)
+
+
+def test_front_end_matches_oracle_on_synthetic():
+    for net in [synth.config_small(), synth.config_large(), synth.config_fixed_heavy(n=2000)]:
+        m = Model(net.bnd(), net.cfg())
+        o = OracleModel(net.bnd(), net.cfg())
+        assert m.names() == o.names() == net.names
+        assert np.array_equal(m.istate(), net.istate)
+        assert np.array_equal(~m.internal(), np.isin(np.arange(net.n), net.readouts))
+
+
+GOOD_BND = "node A { logic = !A; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n"
+
+
+@pytest.mark.parametrize("bnd,cfg,status,msg", [
+    ("node A { logic = A; rate_up = 1; }", "", 1, "rate_down"),
+    ("node A { logic = A; rate_down = 1; }", "", 1, "rate_up"),
+    (GOOD_BND + GOOD_BND, "", 1, "already exists"),
+    ("nod A { logic = A; rate_up = 1; rate_down = 1; }", "", 1, "node"),
+    (GOOD_BND, "B.istate = 1;", 1, "Unknown node"),
+    (GOOD_BND, "max_time = $undefined;", 1, "unknown variable"),
+    (GOOD_BND, "max_time = 5", 1, "syntax error"),
+    (GOOD_BND, "max_time = 5 #;", 1, "invalid character"),
+    ("node A { logic = B; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }", "", 1, "unknown node"),
+    (GOOD_BND, "A.istate = 99999999999;", 1, "out of range"),
+    ("node A { logic = A; rate_up = -1; rate_down = 1; }", "", 1, "non-negative"),
+])
+def test_parse_errors(bnd, cfg, status, msg):
+    with pytest.raises(MbgError) as e:
+        Model(bnd, cfg)
+    assert e.value.status == status
+    assert msg.lower() in str(e.value).lower()
+
+
+def test_more_than_20_readouts_is_a_limit_error():
+    net = synth.random_network(21, 1, 2, seed=1, readouts=21)
+    with pytest.raises(MbgError) as e:
+        Model(net.bnd(), net.cfg())
+    assert e.value.status == 2
+
+
+def test_grammar_details():
+    # keywords are case-insensitive, TRUE/FALSE, comments, precedence of ?: and unary minus
+    bnd = ("Node A { logic = B and NOT C Or false; rate_up = @logic ? 2*3-4/2 : 0; rate_down = @logic ? 0 : -(-1.5e0); }\n"
+           "NODE B { logic = A; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; } // trailing\n"
+           "node C { logic = 1 ? A : B ? C : A; rate_up = @logic ? .5 : 0; rate_down = @logic ? 0 : 2.; }\n")
+    cfg = "[A].istate = 3[1], 1[0];\nB.istate = TRUE;\n$v = 2 + 3 * 4;\nmax_time = $v;\n"
+    m = Model(bnd, cfg)
+    o = OracleModel(bnd, cfg)
+    assert m.names() == ["A", "B", "C"] == o.names()
+    assert m.istate()[0] == np.float32(0.75) == o.istate()[0]
+    assert m.istate()[1] == 1.0
+    assert m.constant("max_time") == 14.0 == o.constant("max_time")
+    assert not m.generic().any()
+
+
+def test_generated_source_is_specialised():
+    net = synth.config_large()
+    src = Model(net.bnd(), net.cfg()).generate_source()
+    assert "#define MBG_NW 16" in src
+    assert "#define MBG_DISCRETE 0" in src
+    assert "mbg_simulate" in src and "mbg_philox" in src
+    b, c = model_text("cellcycle")
+    src = Model(b, c).generate_source(trace=True)
+    assert "#define MBG_NW 1" in src and "#define MBG_DISCRETE 1" in src and "#define MBG_TRACE 1" in src
+
+
+def test_cli_usage_and_parse_errors(tmp_path):
+    r = subprocess.run([CLI_PATH], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage: MaBoSSG [-o prefix] bnd_file cfg_file" in r.stdout
+    r = subprocess.run([CLI_PATH, "-x", "a", "b", "c"], capture_output=True, text=True)
+    assert r.returncode == 1
+    bad = tmp_path / "bad.bnd"
+    bad.write_text("node A {")
+    cfg = tmp_path / "a.cfg"
+    cfg.write_text("")
+    r = subprocess.run([CLI_PATH, str(bad), str(cfg)], capture_output=True, text=True)
+    assert r.returncode == 1 and "syntax error" in r.stderr
+    r = subprocess.run([CLI_PATH, str(tmp_path / "missing.bnd"), str(cfg)], capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot open" in r.stderr
