@@ -254,6 +254,8 @@ class Context:
 
     def run(self, model: Model, n_trajectories=0, count=0, first_id=0, seed=None, variant_id=0, trace=False,
             block=0, grid=0, fetch=True) -> RunResult | None:
+        if count and not n_trajectories:
+            n_trajectories = count  # a partial run is normalised by its own count unless told otherwise
         p = _RunParams(n_trajectories, first_id, count, 0 if seed is None else seed, int(seed is None),
                        variant_id, int(trace), block, grid)
         self._check(lib().mbg_run(self._h, model.handle, C.byref(p)))
