@@ -103,8 +103,9 @@ typedef struct mbg_run_info {
     int windows, readouts, prob_exponent, th_exponent, rate_exponent, state_words;
     int discrete;
     uint64_t n_trajectories, simulated, transitions, fixed_point_trajectories;
-    double kernel_ms;
+    double kernel_ms;          /* mbg_simulate time (HIP events), all batches */
     float time_tick, max_time;
+    int block, tables_in_lds, lds_bytes, blocks_per_cu;  /* launch shape chosen for the run */
 } mbg_run_info;
 mbg_status mbg_get_run_info(const mbg_ctx* c, mbg_run_info* out);
 

@@ -681,6 +681,7 @@ struct orc_model {
     int N = 0, NW = 0;
     std::vector<int> readouts;
     int E = 0;
+    int sigma = 0;  // entropy fixed-point exponent
     size_t max_stack = 8;
 
     float rate_f32(int i, const uint64_t* x, float* st) const {
@@ -711,12 +712,14 @@ static void check_rate(float v, const std::string& n) {
 // 2^-E when possible, capped so that N * max_rate * 2^E < 2^62.
 static void compute_exponent(orc_model& m) {
     std::vector<float> vals;
+    std::vector<std::vector<float>> node_vals(m.N);
     std::vector<float> st(m.max_stack + 4);
     for (int i = 0; i < m.N; i++) {
         const NodeC& c = m.nc[i];
         if (c.fast) {
             if (c.up != 0.f) vals.push_back(c.up);
             if (c.down != 0.f) vals.push_back(c.down);
+            node_vals[i] = {c.up, c.down};
             check_rate(c.up, m.d.nodes[i].name);
             check_rate(c.down, m.d.nodes[i].name);
             continue;
@@ -732,10 +735,12 @@ static void compute_exponent(orc_model& m) {
             float v = m.rate_f32(i, x.data(), st.data());
             check_rate(v, m.d.nodes[i].name);
             if (v != 0.f) vals.push_back(v);
+            node_vals[i].push_back(v);
         }
     }
     if (vals.empty()) {
         m.E = 0;
+        m.sigma = 62;
         return;
     }
     int e_exact = INT_MIN;
@@ -753,6 +758,20 @@ static void compute_exponent(orc_model& m) {
     frexp((double)vmax, &ev);
     int en = ceil_log2_u64((uint64_t)m.N);
     m.E = std::min(e_exact, 62 - ev - en);
+    // entropy exponent: the largest possible sum of r log2 r over the readouts stays below 2^62
+    double tmax = 0.0;
+    for (int j : m.readouts) {
+        double mx = 0.0;
+        for (float v : node_vals[j]) {
+            const uint64_t r = m.to_fixed(v);
+            if (!r) continue;
+            const double rd = (double)r;
+            const double t = rd * log2_impl(rd);
+            if (t > mx) mx = t;
+        }
+        tmax = tmax + mx;
+    }
+    m.sigma = tmax < 1.0 ? 62 : 62 - ceil_log2_d(tmax);
 }
 
 static void build(orc_model& m) {
@@ -822,22 +841,25 @@ namespace {
 // Entropy of the readout rates (reference: src/generator.cpp:146-175), in FP64
 // with the project's log2, over fixed-point rates:
 //   TH = log2(R) - (sum_j r_j log2 r_j) / R,  0 when fewer than 2 readouts are active.
+// with S = sum_j rint(r_j log2(r_j) 2^sigma) an exact integer (sigma from the largest
+// possible sum, see compute_exponent), so that it can be maintained incrementally.
+static uint64_t entropy_term(int sigma, uint64_t r) {
+    if (!r) return 0;
+    const double rd = (double)r;
+    return (uint64_t)nearbyint(ldexp(rd * log2_impl(rd), sigma));
+}
+
 static double readout_entropy(const orc_model& m, const uint64_t* r) {
-    uint64_t R = 0;
+    uint64_t R = 0, S = 0;
     int act = 0;
     for (int j : m.readouts) {
         R += r[j];
         act += r[j] != 0;
+        S += entropy_term(m.sigma, r[j]);
     }
     if (act < 2) return 0.0;
-    double S = 0.0;
-    for (int j : m.readouts)
-        if (r[j]) {
-            double rd = (double)r[j];
-            S = S + rd * log2_impl(rd);
-        }
-    double Rd = (double)R;
-    double th = log2_impl(Rd) - S / Rd;
+    const double Rd = (double)R;
+    const double th = log2_impl(Rd) - ldexp((double)S, -m.sigma) / Rd;
     return th < 0.0 ? 0.0 : th;
 }
 

@@ -1,129 +1,208 @@
-// Persistent Gillespie trajectory kernel + statistics stage for gfx950.
+// Trajectory kernels + statistics stage for gfx950, compiled at run time by hipRTC.
 //
-// Compiled at run time by hipRTC.  The code generator prepends, per model:
-//   MBG_NW       64-bit state words per trajectory (ceil(N / 64))
-//   MBG_BLOCK    threads per block (a multiple of 64)
-//   MBG_DISCRETE 1 for discrete_time models
-//   MBG_TRACE    1 to record per-trajectory summaries (flip count, flip hash)
+// The code generator prepends, per model:
+//   MBG_NW          64-bit state words per trajectory (ceil(N / 64))
+//   MBG_BLOCK       threads per block of mbg_simulate (a multiple of 64)
+//   MBG_DISCRETE    1 for discrete_time models
+//   MBG_TRACE       1 to record per-trajectory summaries (flip count, flip hash)
+//   MBG_GENERIC     1 if some node has a generic (tabulated) rate expression
+//   MBG_TABLES_LDS  1 to stage the model tables in LDS (they fit), 0 to read them from HBM
+//   MBG_MAX_IN      largest number of inputs of a node (gathers are unrolled up to 6)
 // and mbg_math.h.
 //
-// One trajectory per lane (reference: one CUDA thread per trajectory,
-// src/jit_kernels/simulation.cu:77-164).  Unlike the reference's O(N) rate
-// recompute and linear flip scan per step (src/generator.cpp:124-144,
-// simulation.cu:9-21), each lane keeps
+// One trajectory per lane (the reference runs one CUDA thread per trajectory,
+// src/jit_kernels/simulation.cu:77-164).  Unlike the reference's O(N) rate recompute
+// and linear flip scan per step (src/generator.cpp:124-144, simulation.cu:9-21), each
+// lane keeps
 //   x[w]    the Boolean state,
 //   act[w]  the active set (nodes with a non-zero rate),
 //   wsum[w] the exact fixed-point rate sum of each 64-node word,
-// in LDS, lane-interleaved (element w of lane l at w*BLOCK + l, so lanes never
-// share a bank whatever word each one touches).  A transition re-evaluates
-// only the flipped node and its dependents from per-node truth tables and
-// updates the sums with exact integer deltas, so the two-level flip search
-// (word sums, then the active bits of one word) selects exactly the node the
-// reference's linear scan of prefix sums would.
+// in LDS, lane-interleaved (element w of lane l at w*BLOCK + l: no two lanes of a wave
+// share a bank whatever word each touches).  A transition re-evaluates only the flipped
+// node and its dependents from per-node truth tables and applies exact integer deltas,
+// so the two-level search (word sums, then the active bits of one word) selects
+// exactly the node the reference's linear scan of prefix sums would.
 //
-// Statistics (src/jit_kernels/window_average_small.cu:50-136,
-// final_states.cu:9-21, src/statistics/fixed_states.cu:84-125) are
-// accumulated per lane in registers for the current (window, projected state)
-// cell and flushed with 64-bit integer atomics when the cell changes, so the
-// tables are exact sums independent of scheduling.
+// Initial states are built by two throughput-bound pre-pass kernels (one thread per
+// trajectory word) into an HBM staging area, so a lane that starts a trajectory only
+// copies 3*NW words instead of evaluating all N nodes while its wave waits.
+//
+// Statistics (src/jit_kernels/window_average_small.cu:50-136, final_states.cu:9-21,
+// src/statistics/fixed_states.cu:84-125) are accumulated per lane in registers for the
+// current (window, projected state) cell and flushed with 64-bit integer atomics when the
+// cell changes: the tables are exact sums, independent of scheduling.
 
 typedef unsigned int u32;
+typedef unsigned short u16;
 typedef unsigned long long u64;
 
 struct mbg_node {
-    u32 in_off;               // first input in inputs[]
-    u32 dep_off;              // first dependent in deps[]
-    u32 tab_off;              // fast: bit offset into lut[]; generic: index into rtab[]
+    u32 in_off;               // first input in the inputs table
+    u32 dep_off;              // first dependent in the deps table
+    u32 tab_off;              // fast: bit offset into lut; generic: index into rtab
     unsigned short dep_cnt;   // dependents (nodes whose rate reads this node, itself included)
     unsigned char in_cnt;     // inputs other than the node itself
-    unsigned char flags;      // bit 7: generic rate table; bits 0-6: readout index + 1 (0: internal)
+    unsigned char flags;      // bit 7: generic rate table; bit 6: listed among its own
+                              // dependents; bits 0-5: readout index + 1 (0: internal)
+};
+
+#define F_GEN 0x80u
+#define F_SELF 0x40u
+#define F_RO 0x3fu
+
+// Byte offsets of the model tables inside one blob (global memory, or LDS when staged).
+struct mbg_layout {
+    u32 nodes;     // mbg_node[N]
+    u32 rates;     // u64[2N]: fixed-point [rate_up, rate_down]
+    u32 deps;      // u32[]: node id | (key bit position << 24)
+    u32 inputs;    // u16[]: node ids
+    u32 lut;       // u32[]: logic truth tables, key = x_i | inputs << 1
+    u32 readouts;  // u32[K]: non-internal node ids in node order
+    u32 terms;     // u64[2K]: entropy terms rint(r log2(r) 2^sigma) of each readout's U and D
+    u32 gen_mask;  // u64[NW]: nodes with a generic rate table
+    u32 bytes;     // blob size
+    u32 pad;
 };
 
 struct mbg_desc {
-    const mbg_node* nodes;
-    const u32* inputs;        // node ids
-    const u32* deps;          // node id | (key bit position << 24)
-    const u32* lut;           // logic truth tables, key = x_i | inputs << 1
-    const u64* rates;         // fixed-point [rate_up, rate_down] per node (fast nodes)
-    const u64* rtab;          // fixed-point rate per key (generic nodes)
-    const u64* gen_mask;      // per word: nodes with a generic rate table
+    const char* blob;         // model tables (layout above)
+    const u64* rtab;          // fixed-point rate per key of generic nodes
     const float* istate;      // initial ON probability per node
-    const u32* readouts;      // non-internal node ids in node order
-    const double* ro_lg;      // per readout: U*log2(U), D*log2(D) of the fixed-point rates
+    u64* stage;               // [batch][3][NW]: x, act, wsum of each trajectory's initial state
     u64* prob;                // [W][2^K] fixed-point sojourn time (counts when discrete)
     u64* th;                  // [W] fixed-point transition entropy * time
     u64* fin;                 // [2^K] final projected states
-    u64* fp_states;           // [fp_cap][MBG_NW] fixed-point states, append order
+    u64* fp_states;           // [fp_cap][NW] fixed-point states, append order
     u64* fp_count;
     u64* summ;                // [summ_cap][4] flips, flip hash, final time bits, status
-    u64* work_counter;        // next trajectory to start
-    u64* flip_total;          // transitions taken, all trajectories
+    u64* work_counter;        // next trajectory of the batch to start
+    u64* flip_total;          // transitions taken
     u64 fp_cap;
     u64 summ_cap;
     u64 nsteps;               // discrete: steps per trajectory
-    u64 n_traj;
-    u64 first_id;             // trajectory ids are first_id + work index
+    u64 batch;                // trajectories in this batch
+    u64 batch_first;          // id of the batch's first trajectory
+    u64 first_id;             // id of the run's first trajectory (summary index base)
     u64 seed;
     double maxt, tick, inv_tick, rscale;
+    mbg_layout lay;
     u32 variant_id;
-    int N, K, W, FP, FT;
+    int N, K, W, FP, FT, sigma;
 };
 
-#define LANE_X(w) lane_mem[(0 * MBG_NW + (w)) * MBG_BLOCK + tid]
-#define LANE_A(w) lane_mem[(1 * MBG_NW + (w)) * MBG_BLOCK + tid]
-#define LANE_S(w) lane_mem[(2 * MBG_NW + (w)) * MBG_BLOCK + tid]
+#define TNODE(i) (((const mbg_node*)(tb + D.lay.nodes))[i])
+#define TRATE(i) (((const u64*)(tb + D.lay.rates))[i])
+#define TDEP(i) (((const u32*)(tb + D.lay.deps))[i])
+#define TIN(i) ((u32)((const u16*)(tb + D.lay.inputs))[i])
+#define TLUT(i) (((const u32*)(tb + D.lay.lut))[i])
+#define TRO(i) (((const u32*)(tb + D.lay.readouts))[i])
+#define TTERM(i) (((const u64*)(tb + D.lay.terms))[i])
+#define TGEN(i) (((const u64*)(tb + D.lay.gen_mask))[i])
+
+#define LANE_X(w) lane[(0 * MBG_NW + (w)) * MBG_BLOCK + tid]
+#define LANE_A(w) lane[(1 * MBG_NW + (w)) * MBG_BLOCK + tid]
+#define LANE_S(w) lane[(2 * MBG_NW + (w)) * MBG_BLOCK + tid]
+
+// ------------------------------------------------------------------ pre-pass kernels
+
+// Pass 1, one thread per (trajectory, word): initial state bits.  Node i is ON iff the
+// i-th uniform of the trajectory's stream 0 is <= istate_i (simulation.cu:34-71).
+extern "C" __global__ __launch_bounds__(256) void mbg_init_state(mbg_desc D) {
+    const u64 gid = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (gid >= D.batch * MBG_NW) return;
+    const u64 t = gid / MBG_NW;
+    const int w = (int)(gid - t * MBG_NW);
+    const u32 id = (u32)(D.batch_first + t);
+    u64 word = 0;
+    for (int q = 0; q < 16; q++) {
+        const int i0 = w * 64 + q * 4;
+        if (i0 >= D.N) break;
+        u32 o[4];
+        mbg_philox((u32)(i0 >> 2), 0u, id, D.variant_id, (u32)D.seed, (u32)(D.seed >> 32), o);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = i0 + r;
+            if (i < D.N && mbg_unit_f32(o[r]) <= D.istate[i]) word |= 1ull << (i & 63);
+        }
+    }
+    D.stage[t * 3 * MBG_NW + w] = word;
+}
+
+__device__ __forceinline__ u64 rate_from_key(const mbg_desc& D, const char* tb, const mbg_node& n, u32 i, u32 key) {
+#if MBG_GENERIC
+    if (n.flags & 0x80) return D.rtab[n.tab_off + key];
+#endif
+    const u32 bp = n.tab_off + key;
+    const u32 logic = (TLUT(bp >> 5) >> (bp & 31)) & 1u;
+    const u32 xi = key & 1u;
+    return xi == logic ? 0ull : TRATE(2 * i + xi);
+}
+
+// Pass 2, one thread per (trajectory, word): active set and word rate sum.
+extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
+    const u64 gid = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (gid >= D.batch * MBG_NW) return;
+    const u64 t = gid / MBG_NW;
+    const int w = (int)(gid - t * MBG_NW);
+    const u64* x = D.stage + t * 3 * MBG_NW;
+    const char* tb = D.blob;
+    u64 a = 0, sum = 0;
+    for (int b = 0; b < 64; b++) {
+        const u32 i = (u32)(w * 64 + b);
+        if ((int)i >= D.N) break;
+        const mbg_node n = TNODE(i);
+        u32 key = (u32)(x[i >> 6] >> (i & 63)) & 1u;
+        for (u32 m = 0; m < n.in_cnt; m++) {
+            const u32 j = TIN(n.in_off + m);
+            key |= ((u32)(x[j >> 6] >> (j & 63)) & 1u) << (m + 1);
+        }
+        const u64 r = rate_from_key(D, tb, n, i, key);
+        if (r) {
+            a |= 1ull << b;
+            sum += r;
+        }
+    }
+    D.stage[t * 3 * MBG_NW + MBG_NW + w] = a;
+    D.stage[t * 3 * MBG_NW + 2 * MBG_NW + w] = sum;
+}
+
+// ------------------------------------------------------------------ main kernel
 
 struct Lane {
     const mbg_desc& D;
-    u64* lane_mem;
+    const char* tb;
+    u64* lane;
     u32 tid;
 
     __device__ __forceinline__ u32 xbit(u32 j) const { return (u32)(LANE_X(j >> 6) >> (j & 63)) & 1u; }
 
     __device__ __forceinline__ u32 key_of(const mbg_node& n, u32 i) const {
         u32 key = xbit(i);
-        for (u32 m = 0; m < n.in_cnt; m++) key |= xbit(D.inputs[n.in_off + m]) << (m + 1);
+#if MBG_MAX_IN <= 6
+        // branch-free: reads past a node's inputs hit the next node's entries (or the
+        // table's padding) and are masked off
+#pragma unroll
+        for (u32 m = 0; m < MBG_MAX_IN; m++) {
+            const u32 b = xbit(TIN(n.in_off + m));
+            key |= (m < n.in_cnt ? b : 0u) << (m + 1);
+        }
+#else
+        for (u32 m = 0; m < n.in_cnt; m++) key |= xbit(TIN(n.in_off + m)) << (m + 1);
+#endif
         return key;
     }
 
-    __device__ __forceinline__ u64 rate_of(const mbg_node& n, u32 i, u32 key) const {
-        if (n.flags & 0x80) return D.rtab[n.tab_off + key];
-        const u32 bp = n.tab_off + key;
-        const u32 logic = (D.lut[bp >> 5] >> (bp & 31)) & 1u;
-        const u32 xi = key & 1u;
-        return xi == logic ? 0ull : D.rates[2 * i + xi];
-    }
-
-    // Shannon entropy (base 2) of the readout rates (src/generator.cpp:146-175), FP64.
-    __device__ double entropy() const {
-        u64 R = 0;
-        int act = 0;
-        double S = 0.0;
-        for (int j = 0; j < D.K; j++) {
-            const u32 n = D.readouts[j];
-            const mbg_node nd = D.nodes[n];
-            u64 r;
-            double term;
-            if (nd.flags & 0x80) {
-                r = rate_of(nd, n, key_of(nd, n));
-                const double rd = (double)r;
-                term = rd * mbg_log2(rd);
-            } else {
-                const u32 a = (u32)(LANE_A(n >> 6) >> (n & 63)) & 1u;
-                const u32 xi = xbit(n);
-                r = a ? D.rates[2 * n + xi] : 0ull;
-                term = D.ro_lg[2 * j + xi];
-            }
-            if (r) {
-                R += r;
-                act++;
-                S = S + term;
-            }
+    // Entropy term rint(r log2(r) 2^sigma) of readout ro (index) at rate r produced with
+    // own state bit xi: a table entry for fast nodes, computed for generic ones.
+    __device__ __forceinline__ u64 term(const mbg_node& n, u32 ro, u32 xi, u64 r) const {
+        if (!r) return 0ull;
+#if MBG_GENERIC
+        if (n.flags & F_GEN) {
+            const double rd = (double)r;
+            return (u64)rint(ldexp(rd * mbg_log2(rd), D.sigma));
         }
-        if (act < 2) return 0.0;
-        const double Rd = (double)R;
-        const double v = mbg_log2(Rd) - S / Rd;
-        return v < 0.0 ? 0.0 : v;
+#endif
+        return TTERM(2 * ro + xi);
     }
 };
 
@@ -131,21 +210,38 @@ __device__ __forceinline__ void flush_cell(const mbg_desc& D, int w, u32 s, u64 
     if (v) atomicAdd(D.prob + ((u64)w << D.K) + s, v);
 }
 
+#define DEP_BATCH 4
+
 extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D) {
-    __shared__ u64 lane_mem[3 * MBG_NW * MBG_BLOCK];
+    extern __shared__ u64 smem[];
+#if MBG_TABLES_LDS
+    // stage the model tables in LDS once per (persistent) block
+    {
+        const u32 words = D.lay.bytes / 8;
+        const u64* src = (const u64*)D.blob;
+        for (u32 i = threadIdx.x; i < words; i += MBG_BLOCK) smem[i] = src[i];
+        __syncthreads();
+    }
+    const char* tb = (const char*)smem;
+    u64* lane = smem + D.lay.bytes / 8;
+#else
+    const char* tb = D.blob;
+    u64* lane = smem;
+#endif
     const u32 tid = threadIdx.x;
-    const u32 lane = tid & 63u;
-    Lane L{D, lane_mem, tid};
+    const u32 lid = tid & 63u;
+    Lane L{D, tb, lane, tid};
     const u32 key0 = (u32)D.seed, key1 = (u32)(D.seed >> 32);
 
     bool have = false, exhausted = false;
-    u64 id = 0, total = 0, k = 0, acc_p = 0, acc_th = 0;
+    u64 id = 0, total = 0, k = 0, acc_p = 0, acc_th = 0, flips = 0;
+    u64 ro_R = 0, ro_S = 0;  // readout rate sum and entropy-term sum (exact)
+    int ro_act = 0;          // readouts with a non-zero rate
     double t = 0.0, th = 0.0;
     u32 s = 0, acc_s = 0;
     int cur_w = -1;
     bool dirty = true;
     u32 d4[4] = {0u, 0u, 0u, 0u};
-    u64 flips = 0;
 #if MBG_TRACE
     u64 hash = 0;
 #endif
@@ -157,48 +253,42 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
         if (mask) {
             const u32 leader = (u32)__ffsll((long long)mask) - 1u;
             u64 base = 0;
-            if (lane == leader) base = atomicAdd(D.work_counter, (u64)__popcll(mask));
+            if (lid == leader) base = atomicAdd(D.work_counter, (u64)__popcll(mask));
             base = __shfl(base, (int)leader);
             if (need) {
-                const u64 g = base + (u64)__popcll(mask & ((1ull << lane) - 1ull));
-                if (g < D.n_traj) {
+                const u64 g = base + (u64)__popcll(mask & ((1ull << lid) - 1ull));
+                if (g < D.batch) {
                     have = true;
-                    id = D.first_id + g;
-                    // initial state: draw i of the trajectory's stream 0
-                    // (src/jit_kernels/simulation.cu:34-71)
-                    for (int w = 0; w < MBG_NW; w++) {
-                        u64 word = 0;
-                        for (int q = 0; q < 16; q++) {
-                            const int i0 = w * 64 + q * 4;
-                            if (i0 >= D.N) break;
-                            u32 o[4];
-                            mbg_philox((u32)(i0 >> 2), 0u, (u32)id, D.variant_id, key0, key1, o);
-                            for (int r = 0; r < 4; r++) {
-                                const int i = i0 + r;
-                                if (i < D.N && mbg_unit_f32(o[r]) <= D.istate[i]) word |= 1ull << (i & 63);
-                            }
-                        }
-                        LANE_X(w) = word;
-                    }
+                    id = D.batch_first + g;
+                    const u64* st = D.stage + g * 3 * MBG_NW;
                     total = 0;
+#pragma unroll
                     for (int w = 0; w < MBG_NW; w++) {
-                        u64 a = 0, sum = 0;
-                        for (int b = 0; b < 64; b++) {
-                            const u32 i = (u32)(w * 64 + b);
-                            if ((int)i >= D.N) break;
-                            const mbg_node nd = D.nodes[i];
-                            const u64 r = L.rate_of(nd, i, L.key_of(nd, i));
-                            if (r) {
-                                a |= 1ull << b;
-                                sum += r;
-                            }
-                        }
-                        LANE_A(w) = a;
-                        LANE_S(w) = sum;
-                        total += sum;
+                        const u64 xw = st[w], aw = st[MBG_NW + w], sw = st[2 * MBG_NW + w];
+                        LANE_X(w) = xw;
+                        LANE_A(w) = aw;
+                        LANE_S(w) = sw;
+                        total += sw;
                     }
                     s = 0;
-                    for (int j = 0; j < D.K; j++) s |= L.xbit(D.readouts[j]) << j;
+                    ro_R = 0;
+                    ro_S = 0;
+                    ro_act = 0;
+                    for (int j = 0; j < D.K; j++) {
+                        const u32 n = TRO(j);
+                        const u32 xi = L.xbit(n);
+                        s |= xi << j;
+                        const mbg_node nd = TNODE(n);
+                        u64 r;
+#if MBG_GENERIC
+                        if (nd.flags & F_GEN) r = rate_from_key(D, tb, nd, n, L.key_of(nd, n));
+                        else
+#endif
+                            r = ((LANE_A(n >> 6) >> (n & 63)) & 1ull) ? TRATE(2 * n + xi) : 0ull;
+                        ro_R += r;
+                        ro_S += L.term(nd, (u32)j, xi, r);
+                        ro_act += r != 0;
+                    }
                     t = 0.0;
                     k = 0;
                     cur_w = -1;
@@ -236,7 +326,14 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
 #endif
             if (t1 > D.maxt) t1 = D.maxt;
             if (dirty) {
-                th = L.entropy();
+                // Shannon entropy (base 2) of the readout rates (src/generator.cpp:146-175):
+                // TH = log2 R - S 2^-sigma / R, 0 with fewer than two non-zero readout rates
+                th = 0.0;
+                if (ro_act >= 2) {
+                    const double Rd = (double)ro_R;
+                    th = mbg_log2(Rd) - ldexp((double)ro_S, -D.sigma) / Rd;
+                    if (th < 0.0) th = 0.0;
+                }
                 dirty = false;
             }
             th_use = th;
@@ -342,49 +439,114 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
         u64 rem = target - acc;
         u64 m = LANE_A(wsel);
         const u64 xw = LANE_X(wsel);
-        const u64 gm = D.gen_mask[wsel];
+#if MBG_GENERIC
+        const u64 gm = TGEN(wsel);
+#endif
+        const u32 wbase = (u32)wsel * 64u;
         // guard for an inconsistent word (never taken when the sums are exact): last active node
-        u32 j = m ? (u32)(wsel * 64 + 63 - __clzll((long long)m)) : (u32)(wsel * 64);
+        u32 j = m ? (u32)(wsel * 64 + 63 - __clzll((long long)m)) : wbase;
+        u64 r_sel = 0;
         while (m) {
-            const u32 b = (u32)__builtin_ctzll(m);
-            const u32 cand = (u32)wsel * 64u + b;
-            u64 r;
-            if ((gm >> b) & 1ull) {
-                const mbg_node nd = D.nodes[cand];
-                r = L.rate_of(nd, cand, L.key_of(nd, cand));
-            } else {
-                r = D.rates[2 * cand + ((xw >> b) & 1ull)];
+            // two candidates per round trip to the rate table
+            const u32 b0 = (u32)__builtin_ctzll(m);
+            const u64 m1 = m & (m - 1ull);
+            const u32 b1 = m1 ? (u32)__builtin_ctzll(m1) : b0;
+            u64 r0 = TRATE(2 * (wbase + b0) + ((xw >> b0) & 1ull));
+            u64 r1 = TRATE(2 * (wbase + b1) + ((xw >> b1) & 1ull));
+#if MBG_GENERIC
+            if ((gm >> b0) & 1ull) {
+                const mbg_node nd = TNODE(wbase + b0);
+                r0 = rate_from_key(D, tb, nd, wbase + b0, L.key_of(nd, wbase + b0));
             }
-            if (rem < r) {
-                j = cand;
+            if ((gm >> b1) & 1ull) {
+                const mbg_node nd = TNODE(wbase + b1);
+                r1 = rate_from_key(D, tb, nd, wbase + b1, L.key_of(nd, wbase + b1));
+            }
+#endif
+            if (rem < r0) {
+                j = wbase + b0;
+                r_sel = r0;
                 break;
             }
-            rem -= r;
-            m &= m - 1ull;
+            rem -= r0;
+            if (!m1) break;
+            if (rem < r1) {
+                j = wbase + b1;
+                r_sel = r1;
+                break;
+            }
+            rem -= r1;
+            m = m1 & (m1 - 1ull);
         }
 
-        // ---- flip j, re-evaluate its dependents, update sums exactly ----
-        LANE_X(j >> 6) = LANE_X(j >> 6) ^ (1ull << (j & 63));
-        const mbg_node nj = D.nodes[j];
-        for (u32 q = 0; q < nj.dep_cnt; q++) {
-            const u32 dp = D.deps[nj.dep_off + q];
-            const u32 kk = dp & 0xffffffu;
-            const u32 pos = dp >> 24;
-            const mbg_node nk = D.nodes[kk];
-            const u32 key = L.key_of(nk, kk);
-            const u64 rn = L.rate_of(nk, kk, key);
-            const u64 ro = L.rate_of(nk, kk, key ^ (1u << pos));
-            if (rn != ro) {
-                const u32 wk = kk >> 6;
-                const u64 bit = 1ull << (kk & 63);
-                LANE_S(wk) = LANE_S(wk) + (rn - ro);
-                total += rn - ro;
-                const u64 a = LANE_A(wk);
-                LANE_A(wk) = rn ? (a | bit) : (a & ~bit);
-                if (nk.flags & 0x7f) dirty = true;
+        // ---- flip j and update the exact sums ----
+        const u32 xj_old = (u32)(xw >> (j & 63)) & 1u;
+        LANE_X(j >> 6) = xw ^ (1ull << (j & 63));
+        const mbg_node nj = TNODE(j);
+        if (!(nj.flags & F_SELF)) {
+            // fast node that does not read itself: it was active, it now agrees with its
+            // logic, so its rate drops from r_sel to 0
+            const u32 wj = j >> 6;
+            LANE_S(wj) = LANE_S(wj) - r_sel;
+            total -= r_sel;
+            LANE_A(wj) = LANE_A(wj) & ~(1ull << (j & 63));
+            if (nj.flags & F_RO) {
+                ro_R -= r_sel;
+                ro_S -= L.term(nj, (nj.flags & F_RO) - 1u, xj_old, r_sel);
+                ro_act -= 1;
+                dirty = true;
             }
         }
-        if (nj.flags & 0x7f) s ^= 1u << ((nj.flags & 0x7f) - 1);
+        // re-evaluate the dependents (the flipped node too when it reads itself or is generic)
+        for (u32 q0 = 0; q0 < nj.dep_cnt; q0 += DEP_BATCH) {
+            u32 kk[DEP_BATCH], xo[DEP_BATCH], xn[DEP_BATCH];
+            u64 rn[DEP_BATCH], rp[DEP_BATCH];
+            mbg_node nk[DEP_BATCH];
+#pragma unroll
+            for (int u = 0; u < DEP_BATCH; u++) {
+                const bool ok = q0 + u < nj.dep_cnt;
+                const u32 dp = TDEP(nj.dep_off + (ok ? q0 + u : 0u));
+                kk[u] = dp & 0xffffffu;
+                const u32 pos = dp >> 24;
+                nk[u] = TNODE(kk[u]);
+                const u32 key = L.key_of(nk[u], kk[u]);
+                xn[u] = key & 1u;
+                xo[u] = pos == 0 ? xn[u] ^ 1u : xn[u];
+#if MBG_GENERIC
+                if (nk[u].flags & F_GEN) {
+                    rn[u] = D.rtab[nk[u].tab_off + key];
+                    rp[u] = D.rtab[nk[u].tab_off + (key ^ (1u << pos))];
+                } else
+#endif
+                {
+                    const u32 bp = nk[u].tab_off + key;
+                    const u32 logic = (TLUT(bp >> 5) >> (bp & 31)) & 1u;
+                    const u32 a_old = (u32)(LANE_A(kk[u] >> 6) >> (kk[u] & 63)) & 1u;
+                    rn[u] = xn[u] != logic ? TRATE(2 * kk[u] + xn[u]) : 0ull;
+                    rp[u] = a_old ? TRATE(2 * kk[u] + xo[u]) : 0ull;
+                }
+                if (!ok) rn[u] = rp[u];
+            }
+#pragma unroll
+            for (int u = 0; u < DEP_BATCH; u++) {
+                if (rn[u] != rp[u]) {
+                    const u32 wk = kk[u] >> 6;
+                    const u64 bit = 1ull << (kk[u] & 63);
+                    LANE_S(wk) = LANE_S(wk) + (rn[u] - rp[u]);
+                    total += rn[u] - rp[u];
+                    const u64 a = LANE_A(wk);
+                    LANE_A(wk) = rn[u] ? (a | bit) : (a & ~bit);
+                    if (nk[u].flags & F_RO) {
+                        const u32 ro = (nk[u].flags & F_RO) - 1u;
+                        ro_R += rn[u] - rp[u];
+                        ro_S += L.term(nk[u], ro, xn[u], rn[u]) - L.term(nk[u], ro, xo[u], rp[u]);
+                        ro_act += (int)(rn[u] != 0) - (int)(rp[u] != 0);
+                        dirty = true;
+                    }
+                }
+            }
+        }
+        if (nj.flags & F_RO) s ^= 1u << ((nj.flags & F_RO) - 1u);
         flips++;
 #if MBG_TRACE
         hash = (hash ^ (u64)j) * 0x100000001b3ull;

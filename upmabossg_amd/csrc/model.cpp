@@ -115,6 +115,12 @@ void check_rate(float v, const std::string& node) {
 
 }  // namespace
 
+uint64_t entropy_term(int sigma, uint64_t r) {
+    if (!r) return 0;
+    const double rd = (double)r;
+    return (uint64_t)std::nearbyint(std::ldexp(rd * mbg_log2(rd), sigma));
+}
+
 int ceil_log2_u64(uint64_t v) {
     int e = 0;
     while (e < 64 && (1ull << e) < v) e++;
@@ -132,7 +138,7 @@ CompiledModel compile_model(const ParsedModel& pm) {
     CompiledModel cm;
     cm.N = (int)pm.nodes.size();
     if (cm.N == 0) throw ParseError("model has no nodes");
-    if (cm.N >= (1 << 24)) throw ParseError("too many nodes");
+    if (cm.N > 65535) throw ParseError("too many nodes (limit 65535)");
     cm.NW = (cm.N + 63) / 64;
     cm.discrete = pm.constant("discrete_time") != 0.f;
     cm.max_time = pm.constant("max_time");
@@ -173,8 +179,9 @@ CompiledModel compile_model(const ParsedModel& pm) {
             collect_nodes(pm, i, nd.attr("rate_up"), refs);
             collect_nodes(pm, i, nd.attr("rate_down"), refs);
         }
-        refs.erase(i);
+        const bool self_ref = refs.erase(i) > 0;
         const int cnt = (int)refs.size();
+        cm.max_inputs = std::max(cm.max_inputs, cnt);
         if (cnt > (fast ? kMaxInputsFast : kMaxInputsGeneric))
             throw ParseError("node " + nd.name + " reads " + std::to_string(cnt) + " nodes (limit " +
                              std::to_string(fast ? kMaxInputsFast : kMaxInputsGeneric) + ")");
@@ -185,7 +192,13 @@ CompiledModel compile_model(const ParsedModel& pm) {
             cm.inputs.push_back((uint32_t)in[m]);
             dep_lists[in[m]].push_back((uint32_t)i | ((uint32_t)(m + 1) << 24));
         }
-        dep_lists[i].push_back((uint32_t)i);  // every rate reads its own node (key bit 0)
+        // A fast node whose logic does not read itself has rate 0 right after it flips
+        // (it was active: x_i != logic; flipping makes them equal), so the kernel updates it
+        // without a table lookup; every other node re-evaluates itself as a dependent.
+        if (!fast || self_ref) {
+            dep_lists[i].push_back((uint32_t)i);
+            rec.flags |= 0x40;
+        }
         const uint32_t nkeys = 1u << (cnt + 1);
         StateEval ev{pm, i, bits};
         if (fast) {
@@ -203,6 +216,7 @@ CompiledModel compile_model(const ParsedModel& pm) {
             lut_bits += nkeys;
         } else {
             cm.generic[i] = 1;
+            cm.any_generic = true;
             rec.flags |= 0x80;
             cm.gen_mask[i >> 6] |= 1ull << (i & 63);
             rec.tab_off = (uint32_t)cm.gtab.size();
@@ -283,16 +297,32 @@ RateTables make_rate_tables(const CompiledModel& cm, const float* mult_up, const
     }
     t.rtab.resize(gt.size());
     for (size_t k = 0; k < gt.size(); k++) t.rtab[k] = fixed(gt[k]);
-    t.ro_lg.assign(2 * (size_t)cm.K, 0.0);
+    // entropy exponent: the largest possible sum of r log2 r over the readouts stays below 2^62
+    double tmax = 0.0;
     for (int j = 0; j < cm.K; j++) {
         const int n = cm.readouts[j];
-        for (int b = 0; b < 2; b++) {
-            const uint64_t r = t.rates[2 * n + b];
-            if (r && !cm.generic[n]) {
-                const double rd = (double)r;
-                t.ro_lg[2 * j + b] = rd * mbg_log2(rd);
-            }
+        double mx = 0.0;
+        auto visit_r = [&](uint64_t r) {
+            if (!r) return;
+            const double rd = (double)r;
+            const double v = rd * mbg_log2(rd);
+            if (v > mx) mx = v;
+        };
+        if (cm.generic[n]) {
+            const NodeRec& r = cm.nodes[n];
+            for (uint32_t key = 0; key < (1u << (r.in_cnt + 1)); key++) visit_r(t.rtab[r.tab_off + key]);
+        } else {
+            visit_r(t.rates[2 * n]);
+            visit_r(t.rates[2 * n + 1]);
         }
+        tmax = tmax + mx;
+    }
+    t.sigma = tmax < 1.0 ? 62 : 62 - ceil_log2_real(tmax);
+    t.terms.assign(2 * (size_t)cm.K, 0);
+    for (int j = 0; j < cm.K; j++) {
+        const int n = cm.readouts[j];
+        if (cm.generic[n]) continue;
+        for (int b = 0; b < 2; b++) t.terms[2 * j + b] = entropy_term(t.sigma, t.rates[2 * n + b]);
     }
     t.istate.resize(cm.N);
     for (int i = 0; i < cm.N; i++) t.istate[i] = istate ? istate[i] : cm.istate[i];

@@ -24,7 +24,7 @@ struct NodeRec {
     uint32_t tab_off;
     uint16_t dep_cnt;
     uint8_t in_cnt;
-    uint8_t flags;  // bit 7: generic; bits 0-6: readout index + 1
+    uint8_t flags;  // bit 7: generic; bit 6: listed among its own dependents; bits 0-5: readout index + 1
 };
 static_assert(sizeof(NodeRec) == 16, "NodeRec must match the device layout");
 
@@ -48,14 +48,20 @@ struct CompiledModel {
     std::vector<float> up, down;      // fast nodes, FP32
     std::vector<float> gtab;          // generic nodes: FP32 rate per key, at NodeRec::tab_off
     std::vector<float> istate;
+    bool any_generic = false;
+    int max_inputs = 0;               // largest in_cnt
 };
+
+// Entropy term of a fixed-point rate: rint(r * log2(r) * 2^sigma), 0 for r = 0.
+uint64_t entropy_term(int sigma, uint64_t r);
 
 // Per-run (per-variant) numeric tables in the kernel's fixed-point units.
 struct RateTables {
     int E = 0;                        // rate_fixed = rate * 2^E
     std::vector<uint64_t> rates;      // [U, D] per node
     std::vector<uint64_t> rtab;       // generic tables
-    std::vector<double> ro_lg;        // per readout: r*log2(r) for U and D
+    int sigma = 62;                   // entropy term = rint(r * log2(r) * 2^sigma)
+    std::vector<uint64_t> terms;      // per readout: entropy term of its U and D
     std::vector<float> istate;
 };
 

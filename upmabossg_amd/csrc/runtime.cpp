@@ -50,18 +50,18 @@ void copy_err(char* err, size_t n, const std::string& s) {
     }
 }
 
+// Byte offsets of the model tables in one blob; identical to struct mbg_layout in
+// device/mbg_kernel.hip.
+struct Layout {
+    uint32_t nodes, rates, deps, inputs, lut, readouts, terms, gen_mask, bytes, pad;
+};
+
 // Kernel argument block; layout identical to struct mbg_desc in device/mbg_kernel.hip.
 struct Desc {
-    const void* nodes;
-    const void* inputs;
-    const void* deps;
-    const void* lut;
-    const void* rates;
+    const void* blob;
     const void* rtab;
-    const void* gen_mask;
     const void* istate;
-    const void* readouts;
-    const void* ro_lg;
+    void* stage;
     void* prob;
     void* th;
     void* fin;
@@ -70,36 +70,73 @@ struct Desc {
     void* summ;
     void* work_counter;
     void* flip_total;
-    uint64_t fp_cap, summ_cap, nsteps, n_traj, first_id, seed;
+    uint64_t fp_cap, summ_cap, nsteps, batch, batch_first, first_id, seed;
     double maxt, tick, inv_tick, rscale;
+    Layout lay;
     uint32_t variant_id;
-    int N, K, W, FP, FT;
+    int N, K, W, FP, FT, sigma;
 };
 
 struct Shape {
-    int nw, block, discrete, trace;
+    int nw, block, discrete, trace, generic, tables_lds, max_in;
     bool operator<(const Shape& o) const {
-        return std::tie(nw, block, discrete, trace) < std::tie(o.nw, o.block, o.discrete, o.trace);
+        return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in) <
+               std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in);
     }
 };
 
 std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     std::ostringstream os;
-    os << "// Trajectory kernel generated for a model of " << cm.N << " nodes (" << cm.K
+    os << "// Trajectory kernels generated for a model of " << cm.N << " nodes (" << cm.K
        << " non-internal), " << (cm.discrete ? "discrete" : "continuous") << " time.\n";
     os << "#define MBG_NW " << sh.nw << "\n";
     os << "#define MBG_BLOCK " << sh.block << "\n";
     os << "#define MBG_DISCRETE " << sh.discrete << "\n";
     os << "#define MBG_TRACE " << sh.trace << "\n";
+    os << "#define MBG_GENERIC " << sh.generic << "\n";
+    os << "#define MBG_TABLES_LDS " << sh.tables_lds << "\n";
+    os << "#define MBG_MAX_IN " << sh.max_in << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
 }
 
+// Model tables packed into one blob (16-byte aligned segments).
+struct Blob {
+    Layout lay{};
+    std::vector<uint64_t> words;
+};
+
+Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt) {
+    Blob b;
+    std::vector<uint8_t> bytes;
+    auto seg = [&](const void* p, size_t n) -> uint32_t {
+        size_t off = (bytes.size() + 15) & ~(size_t)15;
+        bytes.resize(off + n);
+        if (n) std::memcpy(bytes.data() + off, p, n);
+        return (uint32_t)off;
+    };
+    std::vector<uint16_t> in16(cm.inputs.begin(), cm.inputs.end());
+    in16.resize(in16.size() + 32, 0);  // the kernel's unrolled gathers may read past the last node
+    std::vector<uint32_t> ro(cm.readouts.begin(), cm.readouts.end());
+    b.lay.nodes = seg(cm.nodes.data(), cm.nodes.size() * sizeof(mbg::NodeRec));
+    b.lay.rates = seg(rt.rates.data(), rt.rates.size() * 8);
+    b.lay.deps = seg(cm.deps.data(), cm.deps.size() * 4);
+    b.lay.inputs = seg(in16.data(), in16.size() * 2);
+    b.lay.lut = seg(cm.lut.data(), cm.lut.size() * 4);
+    b.lay.readouts = seg(ro.data(), ro.size() * 4);
+    b.lay.terms = seg(rt.terms.data(), rt.terms.size() * 8);
+    b.lay.gen_mask = seg(cm.gen_mask.data(), cm.gen_mask.size() * 8);
+    bytes.resize((bytes.size() + 15) & ~(size_t)15);
+    b.lay.bytes = (uint32_t)bytes.size();
+    b.words.resize(bytes.size() / 8);
+    std::memcpy(b.words.data(), bytes.data(), bytes.size());
+    return b;
+}
+
 struct Module {
     hipModule_t mod = nullptr;
-    hipFunction_t fn = nullptr;
-    int blocks_per_cu = 1;
+    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr;
 };
 
 template <class T>
@@ -131,6 +168,7 @@ struct DevBuf {
 struct mbg_ctx {
     int device = 0;
     int cus = 0;
+    int lds_per_block = 65536;  // device limit on (static + dynamic) LDS per workgroup
     hipStream_t stream = nullptr;
     std::string err;
     std::map<Shape, Module> modules;
@@ -167,14 +205,47 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
     hiprtcDestroyProgram(&prog);
     Module m;
     hip_check(hipModuleLoadData(&m.mod, code.data()), "hipModuleLoadData");
-    hip_check(hipModuleGetFunction(&m.fn, m.mod, "mbg_simulate"), "hipModuleGetFunction");
-    int nb = 0;
-    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, m.fn, sh.block, 0) != hipSuccess || nb < 1) nb = 1;
-    m.blocks_per_cu = nb;
+    hip_check(hipModuleGetFunction(&m.sim, m.mod, "mbg_simulate"), "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.init_state, m.mod, "mbg_init_state"), "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.init_rates, m.mod, "mbg_init_rates"), "hipModuleGetFunction");
     return c->modules.emplace(sh, m).first->second;
 }
 
-int default_block(const mbg::CompiledModel&) { return 64; }
+size_t lane_lds_bytes(int nw, int block) { return (size_t)3 * nw * block * 8; }
+
+// Block size and table placement: stage the tables in LDS when they fit beside the
+// lanes' state, picking the block size that keeps the most waves resident per CU
+// (160 KiB of LDS per CU on gfx950).
+Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob_bytes, int block_req, int trace) {
+    const size_t cu_lds = 160 * 1024;
+    const int max_in = std::max(1, cm.max_inputs);
+    Shape best{cm.NW, 64, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, 0, max_in};
+    int best_waves = -1;
+    for (int tables = 1; tables >= 0; tables--) {
+        const std::vector<int> blocks = block_req ? std::vector<int>{block_req} : std::vector<int>{256, 192, 128, 64};
+        for (int block : blocks) {
+            const size_t lds = lane_lds_bytes(cm.NW, block) + (tables ? blob_bytes : 0);
+            if (lds > (size_t)c->lds_per_block) continue;
+            const int waves = (int)(cu_lds / lds) * (block / 64);
+            // tables in LDS are worth a modest occupancy loss
+            const int score = tables ? waves * 2 : waves;
+            if (waves > 0 && score > best_waves) {
+                best_waves = score;
+                best = Shape{cm.NW, block, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, tables, max_in};
+            }
+        }
+    }
+    if (best_waves < 0) {
+        if (block_req) throw Error(MBG_ERR_LIMIT, "the requested block does not fit in LDS for this model");
+        throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
+    }
+    return best;
+}
+
+void launch(hipFunction_t f, uint64_t grid, int block, size_t smem, hipStream_t s, Desc& D, const char* what) {
+    void* args[] = {&D};
+    hip_check(hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, (unsigned)smem, s, args, nullptr), what);
+}
 
 void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     const mbg::CompiledModel& cm = m->cm;
@@ -184,14 +255,20 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     const uint64_t count = p->count ? p->count : n;
     if (n == 0 || count == 0) throw Error(MBG_ERR_ARG, "no trajectories to simulate");
     if (p->first_id + count > (1ull << 32)) throw Error(MBG_ERR_ARG, "trajectory ids must stay below 2^32");
-    const int block = p->block ? p->block : default_block(cm);
-    if (block <= 0 || block % 64 || block > 1024) throw Error(MBG_ERR_ARG, "block must be a positive multiple of 64, at most 1024");
-    const Shape sh{cm.NW, block, cm.discrete ? 1 : 0, p->trace ? 1 : 0};
-    const Module& mod = get_module(c, cm, sh);
+    if (p->block && (p->block % 64 || p->block > 1024)) throw Error(MBG_ERR_ARG, "block must be a positive multiple of 64, at most 1024");
 
     const mbg::RateTables rt = mbg::make_rate_tables(cm, m->mult_up.empty() ? nullptr : m->mult_up.data(),
                                                      m->mult_down.empty() ? nullptr : m->mult_down.data(),
                                                      m->istate.empty() ? nullptr : m->istate.data());
+    const Blob blob = pack_blob(cm, rt);
+    const Shape sh = choose_shape(c, cm, blob.lay.bytes, p->block, p->trace);
+    const Module& mod = get_module(c, cm, sh);
+    const size_t smem = lane_lds_bytes(cm.NW, sh.block) + (sh.tables_lds ? blob.lay.bytes : 0);
+    int blocks_per_cu = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, mod.sim, sh.block, smem) != hipSuccess ||
+        blocks_per_cu < 1)
+        blocks_per_cu = 1;
+
     const int W = (int)std::ceil(cm.max_time / cm.time_tick);  // window_average_small.cpp:26
     const uint64_t S = 1ull << cm.K;
     int FP, FT;
@@ -207,24 +284,16 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     const uint64_t nsteps = ratio >= 2.f ? (uint64_t)std::floor(ratio) : 1;
     // Fixed points: one slot per simulated trajectory, within a 16 GiB budget.
     const uint64_t fp_cap = std::min<uint64_t>(count, (16ull << 30) / (8ull * (uint64_t)cm.NW));
+    // Initial states are staged per batch, within a 8 GiB budget.
+    const uint64_t batch_max = std::max<uint64_t>(1, std::min<uint64_t>(count, (8ull << 30) / (24ull * (uint64_t)cm.NW)));
 
     hipStream_t s = c->stream;
-    DevBuf<mbg::NodeRec> d_nodes;
-    DevBuf<uint32_t> d_inputs, d_deps, d_lut, d_ro;
-    DevBuf<uint64_t> d_rates, d_rtab, d_gm, d_prob, d_th, d_fin, d_fp, d_fpc, d_summ, d_work, d_flips;
+    DevBuf<uint64_t> d_blob, d_rtab, d_stage, d_prob, d_th, d_fin, d_fp, d_fpc, d_summ, d_work, d_flips;
     DevBuf<float> d_istate;
-    DevBuf<double> d_rolg;
-    d_nodes.upload(cm.nodes, s);
-    d_inputs.upload(cm.inputs, s);
-    d_deps.upload(cm.deps, s);
-    d_lut.upload(cm.lut, s);
-    std::vector<uint32_t> ro(cm.readouts.begin(), cm.readouts.end());
-    d_ro.upload(ro, s);
-    d_rates.upload(rt.rates, s);
+    d_blob.upload(blob.words, s);
     d_rtab.upload(rt.rtab, s);
-    d_gm.upload(cm.gen_mask, s);
     d_istate.upload(rt.istate, s);
-    d_rolg.upload(rt.ro_lg, s);
+    d_stage.alloc((size_t)batch_max * 3 * cm.NW);
     d_prob.zero((size_t)W * S, s);
     d_th.zero((size_t)W, s);
     d_fin.zero((size_t)S, s);
@@ -232,20 +301,14 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     d_fpc.zero(1, s);
     const uint64_t summ_cap = p->trace ? count : 0;
     d_summ.zero((size_t)summ_cap * 4, s);
-    d_work.zero(1, s);
+    d_work.alloc(1);
     d_flips.zero(1, s);
 
     Desc D{};
-    D.nodes = d_nodes.p;
-    D.inputs = d_inputs.p;
-    D.deps = d_deps.p;
-    D.lut = d_lut.p;
-    D.rates = d_rates.p;
+    D.blob = d_blob.p;
     D.rtab = d_rtab.p;
-    D.gen_mask = d_gm.p;
     D.istate = d_istate.p;
-    D.readouts = d_ro.p;
-    D.ro_lg = d_rolg.p;
+    D.stage = d_stage.p;
     D.prob = d_prob.p;
     D.th = d_th.p;
     D.fin = d_fin.p;
@@ -257,33 +320,42 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     D.fp_cap = fp_cap;
     D.summ_cap = summ_cap;
     D.nsteps = nsteps;
-    D.n_traj = count;
     D.first_id = p->first_id;
     D.seed = p->use_model_seed ? (uint64_t)cm.seed : p->seed;
     D.maxt = (double)cm.max_time;
     D.tick = (double)cm.time_tick;
     D.inv_tick = 1.0 / D.tick;
     D.rscale = std::ldexp(1.0, -rt.E);
+    D.lay = blob.lay;
     D.variant_id = p->variant_id;
     D.N = cm.N;
     D.K = cm.K;
     D.W = W;
     D.FP = FP;
     D.FT = FT;
+    D.sigma = rt.sigma;
 
-    uint64_t grid = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)mod.blocks_per_cu;
-    grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (count + block - 1) / block));
-    void* args[] = {&D};
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
-    hip_check(hipEventRecord(e0, s), "hipEventRecord");
-    hip_check(hipModuleLaunchKernel(mod.fn, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, 0, s, args, nullptr),
-              "hipModuleLaunchKernel");
-    hip_check(hipEventRecord(e1, s), "hipEventRecord");
-    hip_check(hipStreamSynchronize(s), "kernel execution");
-    float ms = 0.f;
-    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    float sim_ms = 0.f;
+    for (uint64_t done = 0; done < count; done += D.batch) {
+        D.batch = std::min(batch_max, count - done);
+        D.batch_first = p->first_id + done;
+        hip_check(hipMemsetAsync(d_work.p, 0, 8, s), "hipMemsetAsync");
+        const uint64_t init_threads = D.batch * (uint64_t)cm.NW;
+        launch(mod.init_state, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_state");
+        launch(mod.init_rates, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_rates");
+        uint64_t grid = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)blocks_per_cu;
+        grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (D.batch + sh.block - 1) / sh.block));
+        hip_check(hipEventRecord(e0, s), "hipEventRecord");
+        launch(mod.sim, grid, sh.block, smem, s, D, "launch mbg_simulate");
+        hip_check(hipEventRecord(e1, s), "hipEventRecord");
+        hip_check(hipStreamSynchronize(s), "kernel execution");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        sim_ms += ms;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 
@@ -335,9 +407,13 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     I.simulated = count;
     I.transitions = flips;
     I.fixed_point_trajectories = nfp;
-    I.kernel_ms = ms;
+    I.kernel_ms = sim_ms;
     I.time_tick = cm.time_tick;
     I.max_time = cm.max_time;
+    I.block = sh.block;
+    I.tables_in_lds = sh.tables_lds;
+    I.lds_bytes = (int)smem;
+    I.blocks_per_cu = blocks_per_cu;
     c->have = true;
 }
 
@@ -473,7 +549,9 @@ mbg_status mbg_model_set_variant(mbg_model* m, const float* mult_up, const float
 
 mbg_status mbg_model_generate_source(const mbg_model* m, int trace, char* buf, size_t cap, size_t* len) {
     if (!m) return MBG_ERR_ARG;
-    const std::string s = generate_source(m->cm, Shape{m->cm.NW, default_block(m->cm), m->cm.discrete ? 1 : 0, trace ? 1 : 0});
+    const mbg::CompiledModel& cm = m->cm;
+    const std::string s = generate_source(cm, Shape{cm.NW, 64, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, 0,
+                                                    std::max(1, cm.max_inputs)});
     if (len) *len = s.size();
     if (buf && cap) {
         const size_t k = std::min(cap - 1, s.size());
@@ -493,6 +571,8 @@ mbg_status mbg_create(int device, mbg_ctx** out) {
     c->device = device;
     if (hipSetDevice(device) != hipSuccess) return MBG_ERR_HIP;
     if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return MBG_ERR_HIP;
+    if (hipDeviceGetAttribute(&c->lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        return MBG_ERR_HIP;
     if (hipStreamCreate(&c->stream) != hipSuccess) return MBG_ERR_HIP;
     *out = c.release();
     return MBG_OK;
@@ -512,7 +592,9 @@ mbg_status mbg_compile(mbg_ctx* c, const mbg_model* m, int trace) {
     if (!c || !m) return MBG_ERR_ARG;
     return guarded(c, [&] {
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        get_module(c, m->cm, Shape{m->cm.NW, default_block(m->cm), m->cm.discrete ? 1 : 0, trace ? 1 : 0});
+        const mbg::CompiledModel& cm = m->cm;
+        const mbg::RateTables rt = mbg::make_rate_tables(cm, nullptr, nullptr, nullptr);
+        get_module(c, cm, choose_shape(c, cm, pack_blob(cm, rt).lay.bytes, 0, trace));
     });
 }
 
