@@ -8,6 +8,9 @@
 //   MBG_GENERIC     1 if some node has a generic (tabulated) rate expression
 //   MBG_TABLES_LDS  1 to stage the model tables in LDS (they fit), 0 to read them from HBM
 //   MBG_MAX_IN      largest number of inputs of a node (gathers are unrolled up to 6)
+//   MBG_DEP_BATCH   dependents evaluated per round of independent table reads
+//   MBG_SCAN        candidates of the in-word flip search per round of table reads (2 or 4)
+//   MBG_WSUM_REG    1: the word rate sums live in VGPRs (more lanes fit in LDS), 0: in LDS
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -101,7 +104,21 @@ struct mbg_desc {
 
 #define LANE_X(w) lane[(0 * MBG_NW + (w)) * MBG_BLOCK + tid]
 #define LANE_A(w) lane[(1 * MBG_NW + (w)) * MBG_BLOCK + tid]
+#if MBG_WSUM_REG
+// word sums in registers: every access is an unrolled select over the NW words
+#define LANE_S(w) ws[w]
+#define WS_ADD(w, d)                                                       \
+    do {                                                                   \
+        const u32 w_ = (w);                                                \
+        const u64 d_ = (d);                                                \
+        _Pragma("unroll") for (int i_ = 0; i_ < MBG_NW; i_++) ws[i_] += w_ == (u32)i_ ? d_ : 0ull; \
+    } while (0)
+#define LANE_WORDS 2
+#else
 #define LANE_S(w) lane[(2 * MBG_NW + (w)) * MBG_BLOCK + tid]
+#define WS_ADD(w, d) LANE_S(w) = LANE_S(w) + (d)
+#define LANE_WORDS 3
+#endif
 
 // ------------------------------------------------------------------ pre-pass kernels
 
@@ -210,7 +227,7 @@ __device__ __forceinline__ void flush_cell(const mbg_desc& D, int w, u32 s, u64 
     if (v) atomicAdd(D.prob + ((u64)w << D.K) + s, v);
 }
 
-#define DEP_BATCH 4
+#define DEP_BATCH MBG_DEP_BATCH
 
 extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D) {
     extern __shared__ u64 smem[];
@@ -236,6 +253,11 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
     bool have = false, exhausted = false;
     u64 id = 0, total = 0, k = 0, acc_p = 0, acc_th = 0, flips = 0;
     u64 ro_R = 0, ro_S = 0;  // readout rate sum and entropy-term sum (exact)
+#if MBG_WSUM_REG
+    u64 ws[MBG_NW];
+#pragma unroll
+    for (int w = 0; w < MBG_NW; w++) ws[w] = 0;
+#endif
     int ro_act = 0;          // readouts with a non-zero rate
     double t = 0.0, th = 0.0;
     u32 s = 0, acc_s = 0;
@@ -446,6 +468,41 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
         // guard for an inconsistent word (never taken when the sums are exact): last active node
         u32 j = m ? (u32)(wsel * 64 + 63 - __clzll((long long)m)) : wbase;
         u64 r_sel = 0;
+#if MBG_SCAN == 4
+        while (m) {
+            // four candidates per round trip to the rate table
+            u32 bb[4];
+            u64 rr[4], mm = m;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                bb[c] = mm ? (u32)__builtin_ctzll(mm) : 64u;
+                mm = mm ? (mm & (mm - 1ull)) : 0ull;
+                const u32 b = bb[c] < 64u ? bb[c] : 0u;
+                rr[c] = bb[c] < 64u ? TRATE(2 * (wbase + b) + ((xw >> b) & 1ull)) : 0ull;
+#if MBG_GENERIC
+                if (bb[c] < 64u && ((gm >> b) & 1ull)) {
+                    const mbg_node nd = TNODE(wbase + b);
+                    rr[c] = rate_from_key(D, tb, nd, wbase + b, L.key_of(nd, wbase + b));
+                }
+#endif
+            }
+            bool hit = false;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                if (!hit && bb[c] < 64u) {
+                    if (rem < rr[c]) {
+                        j = wbase + bb[c];
+                        r_sel = rr[c];
+                        hit = true;
+                    } else {
+                        rem -= rr[c];
+                    }
+                }
+            }
+            if (hit) break;
+            m = mm;
+        }
+#else
         while (m) {
             // two candidates per round trip to the rate table
             const u32 b0 = (u32)__builtin_ctzll(m);
@@ -478,6 +535,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
             rem -= r1;
             m = m1 & (m1 - 1ull);
         }
+#endif
 
         // ---- flip j and update the exact sums ----
         const u32 xj_old = (u32)(xw >> (j & 63)) & 1u;
@@ -487,7 +545,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
             // fast node that does not read itself: it was active, it now agrees with its
             // logic, so its rate drops from r_sel to 0
             const u32 wj = j >> 6;
-            LANE_S(wj) = LANE_S(wj) - r_sel;
+            WS_ADD(wj, 0ull - r_sel);
             total -= r_sel;
             LANE_A(wj) = LANE_A(wj) & ~(1ull << (j & 63));
             if (nj.flags & F_RO) {
@@ -532,7 +590,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
                 if (rn[u] != rp[u]) {
                     const u32 wk = kk[u] >> 6;
                     const u64 bit = 1ull << (kk[u] & 63);
-                    LANE_S(wk) = LANE_S(wk) + (rn[u] - rp[u]);
+                    WS_ADD(wk, rn[u] - rp[u]);
                     total += rn[u] - rp[u];
                     const u64 a = LANE_A(wk);
                     LANE_A(wk) = rn[u] ? (a | bit) : (a & ~bit);

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -78,10 +79,11 @@ struct Desc {
 };
 
 struct Shape {
-    int nw, block, discrete, trace, generic, tables_lds, max_in;
+    int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 4, scan = 4, wsum_reg = 0;
     bool operator<(const Shape& o) const {
-        return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in) <
-               std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in);
+        return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg) <
+               std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in, o.dep_batch, o.scan,
+                        o.wsum_reg);
     }
 };
 
@@ -96,6 +98,9 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_GENERIC " << sh.generic << "\n";
     os << "#define MBG_TABLES_LDS " << sh.tables_lds << "\n";
     os << "#define MBG_MAX_IN " << sh.max_in << "\n";
+    os << "#define MBG_DEP_BATCH " << sh.dep_batch << "\n";
+    os << "#define MBG_SCAN " << sh.scan << "\n";
+    os << "#define MBG_WSUM_REG " << sh.wsum_reg << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -211,31 +216,43 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
     return c->modules.emplace(sh, m).first->second;
 }
 
-size_t lane_lds_bytes(int nw, int block) { return (size_t)3 * nw * block * 8; }
+size_t lane_lds_bytes(int nw, int block, int wsum_reg) { return (size_t)(wsum_reg ? 2 : 3) * nw * block * 8; }
 
-// Block size and table placement: stage the tables in LDS when they fit beside the
-// lanes' state, picking the block size that keeps the most waves resident per CU
-// (160 KiB of LDS per CU on gfx950).
+int env_int(const char* name, int dflt) {
+    const char* e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+
+// Launch shape: block size, table placement (LDS or HBM) and word-sum placement (LDS or
+// VGPRs).  Without tuning knobs: stage the tables in LDS when they fit beside the lanes'
+// state and keep the most waves resident per CU (160 KiB of LDS per CU on gfx950).
 Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob_bytes, int block_req, int trace) {
     const size_t cu_lds = 160 * 1024;
     const int max_in = std::max(1, cm.max_inputs);
+    const int wsum_reg = env_int("MBG_TUNE_WSUM_REG", 1) ? 1 : 0;
+    const int tables_env = env_int("MBG_TUNE_TABLES", -1);
+    if (!block_req) block_req = env_int("MBG_TUNE_BLOCK", 0);
     Shape best{cm.NW, 64, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, 0, max_in};
-    int best_waves = -1;
+    best.wsum_reg = wsum_reg;
+    int best_score = -1;
     for (int tables = 1; tables >= 0; tables--) {
-        const std::vector<int> blocks = block_req ? std::vector<int>{block_req} : std::vector<int>{256, 192, 128, 64};
-        for (int block : blocks) {
-            const size_t lds = lane_lds_bytes(cm.NW, block) + (tables ? blob_bytes : 0);
+        if (tables_env >= 0 && tables != tables_env) continue;
+        for (int block = 1024; block >= 64; block -= 64) {
+            if (block_req && block != block_req) continue;
+            const size_t lds = lane_lds_bytes(cm.NW, block, wsum_reg) + (tables ? blob_bytes : 0);
             if (lds > (size_t)c->lds_per_block) continue;
-            const int waves = (int)(cu_lds / lds) * (block / 64);
-            // tables in LDS are worth a modest occupancy loss
-            const int score = tables ? waves * 2 : waves;
-            if (waves > 0 && score > best_waves) {
-                best_waves = score;
-                best = Shape{cm.NW, block, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, tables, max_in};
+            const int waves = std::min(16, (int)(cu_lds / lds) * (block / 64));
+            const int score = tables ? waves * 2 : waves;  // tables in LDS are worth a modest occupancy loss
+            if (waves > 0 && score > best_score) {
+                best_score = score;
+                best.block = block;
+                best.tables_lds = tables;
             }
         }
     }
-    if (best_waves < 0) {
+    best.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 4)));
+    best.scan = env_int("MBG_TUNE_SCAN", 4) == 2 ? 2 : 4;
+    if (best_score < 0) {
         if (block_req) throw Error(MBG_ERR_LIMIT, "the requested block does not fit in LDS for this model");
         throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
     }
@@ -263,7 +280,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     const Blob blob = pack_blob(cm, rt);
     const Shape sh = choose_shape(c, cm, blob.lay.bytes, p->block, p->trace);
     const Module& mod = get_module(c, cm, sh);
-    const size_t smem = lane_lds_bytes(cm.NW, sh.block) + (sh.tables_lds ? blob.lay.bytes : 0);
+    const size_t smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0);
     int blocks_per_cu = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, mod.sim, sh.block, smem) != hipSuccess ||
         blocks_per_cu < 1)
