@@ -11,6 +11,7 @@
 //   MBG_DEP_BATCH   dependents evaluated per round of independent table reads
 //   MBG_SCAN        candidates of the in-word flip search per round of table reads (2 or 4)
 //   MBG_WSUM_REG    1: the word rate sums live in VGPRs (more lanes fit in LDS), 0: in LDS
+//   MBG_RATE32      1: every fixed-point rate fits 32 bits, the rate table is u32
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -56,7 +57,7 @@ struct mbg_node {
 // Byte offsets of the model tables inside one blob (global memory, or LDS when staged).
 struct mbg_layout {
     u32 nodes;     // mbg_node[N]
-    u32 rates;     // u64[2N]: fixed-point [rate_up, rate_down]
+    u32 rates;     // u64[2N] (u32[2N] with MBG_RATE32): fixed-point [rate_up, rate_down]
     u32 deps;      // u32[]: node id | (key bit position << 24)
     u32 inputs;    // u16[]: node ids
     u32 lut;       // u32[]: logic truth tables, key = x_i | inputs << 1
@@ -94,7 +95,11 @@ struct mbg_desc {
 };
 
 #define TNODE(i) (((const mbg_node*)(tb + D.lay.nodes))[i])
+#if MBG_RATE32
+#define TRATE(i) ((u64)((const u32*)(tb + D.lay.rates))[i])
+#else
 #define TRATE(i) (((const u64*)(tb + D.lay.rates))[i])
+#endif
 #define TDEP(i) (((const u32*)(tb + D.lay.deps))[i])
 #define TIN(i) ((u32)((const u16*)(tb + D.lay.inputs))[i])
 #define TLUT(i) (((const u32*)(tb + D.lay.lut))[i])
@@ -155,7 +160,49 @@ __device__ __forceinline__ u64 rate_from_key(const mbg_desc& D, const char* tb, 
     return xi == logic ? 0ull : TRATE(2 * i + xi);
 }
 
-// Pass 2, one thread per (trajectory, word): active set and word rate sum.
+// Pass 2: active set and word rate sums.  A block takes 64 trajectories: their state
+// words are staged in LDS (lane-interleaved), then each wave sweeps whole words for the
+// 64 trajectories, one per lane, so the table reads of a word's 64 nodes are uniform
+// across the wave and the state reads never conflict.  Models above 128 words use the
+// per-(trajectory, word) variant below.
+#define INIT_TRAJ 64
+#if MBG_NW <= 128
+extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
+    __shared__ u64 xs[MBG_NW * INIT_TRAJ];
+    const u32 tid = threadIdx.x;
+    const u32 lane = tid & 63u;
+    const u64 t0 = (u64)blockIdx.x * INIT_TRAJ;
+    for (u32 idx = tid; idx < MBG_NW * INIT_TRAJ; idx += 256) {
+        const u32 w = idx / INIT_TRAJ, l = idx % INIT_TRAJ;
+        xs[idx] = t0 + l < D.batch ? D.stage[(t0 + l) * 3 * MBG_NW + w] : 0ull;
+    }
+    __syncthreads();
+    const u64 t = t0 + lane;
+    const char* tb = D.blob;
+    for (int w = (int)(tid >> 6); w < MBG_NW; w += 4) {
+        u64 a = 0, sum = 0;
+        for (int b = 0; b < 64; b++) {
+            const u32 i = (u32)(w * 64 + b);
+            if ((int)i >= D.N) break;
+            const mbg_node n = TNODE(i);
+            u32 key = (u32)(xs[(i >> 6) * INIT_TRAJ + lane] >> (i & 63)) & 1u;
+            for (u32 m = 0; m < n.in_cnt; m++) {
+                const u32 j = TIN(n.in_off + m);
+                key |= ((u32)(xs[(j >> 6) * INIT_TRAJ + lane] >> (j & 63)) & 1u) << (m + 1);
+            }
+            const u64 r = rate_from_key(D, tb, n, i, key);
+            if (r) {
+                a |= 1ull << b;
+                sum += r;
+            }
+        }
+        if (t < D.batch) {
+            D.stage[t * 3 * MBG_NW + MBG_NW + w] = a;
+            D.stage[t * 3 * MBG_NW + 2 * MBG_NW + w] = sum;
+        }
+    }
+}
+#else
 extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
     const u64 gid = (u64)blockIdx.x * 256u + threadIdx.x;
     if (gid >= D.batch * MBG_NW) return;
@@ -182,6 +229,7 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
     D.stage[t * 3 * MBG_NW + MBG_NW + w] = a;
     D.stage[t * 3 * MBG_NW + 2 * MBG_NW + w] = sum;
 }
+#endif
 
 // ------------------------------------------------------------------ main kernel
 

@@ -79,11 +79,11 @@ struct Desc {
 };
 
 struct Shape {
-    int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 4, scan = 4, wsum_reg = 0;
+    int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 4, scan = 4, wsum_reg = 0, rate32 = 0;
     bool operator<(const Shape& o) const {
-        return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg) <
+        return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32) <
                std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in, o.dep_batch, o.scan,
-                        o.wsum_reg);
+                        o.wsum_reg, o.rate32);
     }
 };
 
@@ -101,6 +101,7 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_DEP_BATCH " << sh.dep_batch << "\n";
     os << "#define MBG_SCAN " << sh.scan << "\n";
     os << "#define MBG_WSUM_REG " << sh.wsum_reg << "\n";
+    os << "#define MBG_RATE32 " << sh.rate32 << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -111,6 +112,12 @@ struct Blob {
     Layout lay{};
     std::vector<uint64_t> words;
 };
+
+bool rates_fit_u32(const mbg::RateTables& rt) {
+    for (uint64_t r : rt.rates)
+        if (r >> 32) return false;
+    return true;
+}
 
 Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt) {
     Blob b;
@@ -125,7 +132,12 @@ Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt) {
     in16.resize(in16.size() + 32, 0);  // the kernel's unrolled gathers may read past the last node
     std::vector<uint32_t> ro(cm.readouts.begin(), cm.readouts.end());
     b.lay.nodes = seg(cm.nodes.data(), cm.nodes.size() * sizeof(mbg::NodeRec));
-    b.lay.rates = seg(rt.rates.data(), rt.rates.size() * 8);
+    if (rates_fit_u32(rt)) {
+        std::vector<uint32_t> r32(rt.rates.begin(), rt.rates.end());
+        b.lay.rates = seg(r32.data(), r32.size() * 4);
+    } else {
+        b.lay.rates = seg(rt.rates.data(), rt.rates.size() * 8);
+    }
     b.lay.deps = seg(cm.deps.data(), cm.deps.size() * 4);
     b.lay.inputs = seg(in16.data(), in16.size() * 2);
     b.lay.lut = seg(cm.lut.data(), cm.lut.size() * 4);
@@ -226,7 +238,8 @@ int env_int(const char* name, int dflt) {
 // Launch shape: block size, table placement (LDS or HBM) and word-sum placement (LDS or
 // VGPRs).  Without tuning knobs: stage the tables in LDS when they fit beside the lanes'
 // state and keep the most waves resident per CU (160 KiB of LDS per CU on gfx950).
-Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob_bytes, int block_req, int trace) {
+Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob_bytes, int block_req, int trace,
+                   int rate32) {
     const size_t cu_lds = 160 * 1024;
     const int max_in = std::max(1, cm.max_inputs);
     const int wsum_reg = env_int("MBG_TUNE_WSUM_REG", 1) ? 1 : 0;
@@ -234,6 +247,7 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
     if (!block_req) block_req = env_int("MBG_TUNE_BLOCK", 0);
     Shape best{cm.NW, 64, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, 0, max_in};
     best.wsum_reg = wsum_reg;
+    best.rate32 = rate32;
     int best_score = -1;
     for (int tables = 1; tables >= 0; tables--) {
         if (tables_env >= 0 && tables != tables_env) continue;
@@ -278,7 +292,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
                                                      m->mult_down.empty() ? nullptr : m->mult_down.data(),
                                                      m->istate.empty() ? nullptr : m->istate.data());
     const Blob blob = pack_blob(cm, rt);
-    const Shape sh = choose_shape(c, cm, blob.lay.bytes, p->block, p->trace);
+    const Shape sh = choose_shape(c, cm, blob.lay.bytes, p->block, p->trace, rates_fit_u32(rt) ? 1 : 0);
     const Module& mod = get_module(c, cm, sh);
     const size_t smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0);
     int blocks_per_cu = 0;
@@ -361,8 +375,10 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
         D.batch_first = p->first_id + done;
         hip_check(hipMemsetAsync(d_work.p, 0, 8, s), "hipMemsetAsync");
         const uint64_t init_threads = D.batch * (uint64_t)cm.NW;
+        // mbg_init_rates: one block per 64 trajectories (per (trajectory, word) above 128 words)
+        const uint64_t rate_blocks = cm.NW <= 128 ? (D.batch + 63) / 64 : (init_threads + 255) / 256;
         launch(mod.init_state, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_state");
-        launch(mod.init_rates, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_rates");
+        launch(mod.init_rates, rate_blocks, 256, 0, s, D, "launch mbg_init_rates");
         uint64_t grid = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)blocks_per_cu;
         grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (D.batch + sh.block - 1) / sh.block));
         hip_check(hipEventRecord(e0, s), "hipEventRecord");
@@ -611,7 +627,7 @@ mbg_status mbg_compile(mbg_ctx* c, const mbg_model* m, int trace) {
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         const mbg::CompiledModel& cm = m->cm;
         const mbg::RateTables rt = mbg::make_rate_tables(cm, nullptr, nullptr, nullptr);
-        get_module(c, cm, choose_shape(c, cm, pack_blob(cm, rt).lay.bytes, 0, trace));
+        get_module(c, cm, choose_shape(c, cm, pack_blob(cm, rt).lay.bytes, 0, trace, rates_fit_u32(rt) ? 1 : 0));
     });
 }
 
