@@ -124,6 +124,44 @@ mbg_status mbg_get_fixed_points(const mbg_ctx* c, uint64_t* states, uint64_t* co
  * node indices, final time (FP64 bits), status (1 finished, 2 fixed point). */
 mbg_status mbg_get_summaries(const mbg_ctx* c, uint64_t* out, size_t count);
 
+/* Readout marginals of the last run: out[W * K] = probability that readout j is ON, averaged
+ * over window w (the window table summed over the projected states with bit j set). */
+mbg_status mbg_get_marginals(const mbg_ctx* c, double* out);
+
+/* ---------------------------------------------------------------- variant sweep */
+
+/* One variant of a sweep: a model (its topology and, through mbg_model_set_variant, its rate
+ * multipliers, knockouts and initial conditions) with its own trajectory count and seed. */
+typedef struct mbg_variant {
+    const mbg_model* model;
+    uint64_t n_trajectories; /* trajectories to simulate (and normalisation count); 0 = sample_count */
+    uint64_t seed;
+    int use_model_seed;      /* 1: seed_pseudorandom of the model */
+    uint32_t variant_id;     /* Philox counter word 3 */
+} mbg_variant;
+
+/* Runs every variant in one launch: the models are grouped by state-word class (1, 2, 4, ...
+ * 64-bit words), one persistent kernel per class drains its variants' work queues, and all
+ * pre-passes and class kernels are captured into one HIP graph (replaces running the
+ * reference's CLI once per variant; README.md:33 lists mutant analysis as future work).
+ * Synchronous.  The variants' tables stay on the device until the next run. */
+mbg_status mbg_run_variants(mbg_ctx* c, const mbg_variant* variants, int count);
+
+typedef struct mbg_sweep_info {
+    int variants, classes;
+    uint64_t trajectories, transitions;
+    double graph_ms;     /* the graph launch (HIP events), pre-passes and kernels included */
+    double setup_ms;     /* host table building, uploads and graph construction */
+} mbg_sweep_info;
+mbg_status mbg_get_sweep_info(const mbg_ctx* c, mbg_sweep_info* out);
+/* Per-variant results of the last sweep (same layouts as the single-run getters). */
+mbg_status mbg_get_variant_info(const mbg_ctx* c, int v, mbg_run_info* out);
+mbg_status mbg_get_variant_probtraj_raw(const mbg_ctx* c, int v, uint64_t* prob, uint64_t* th);
+mbg_status mbg_get_variant_marginals(const mbg_ctx* c, int v, double* out);
+mbg_status mbg_get_variant_final_states(const mbg_ctx* c, int v, uint64_t* counts);
+mbg_status mbg_get_variant_fixed_points(const mbg_ctx* c, int v, uint64_t* states, uint64_t* counts, size_t cap,
+                                        size_t* n);
+
 /* Output files of the reference: <prefix>_probtraj.csv (window_average_small_stats::write_csv,
  * src/statistics/window_average_small.cpp:181-250) and <prefix>_fp.csv
  * (fixed_states_stats::write_csv, src/statistics/fixed_states.cu:154-187). */

@@ -64,6 +64,21 @@ class _RunInfo(C.Structure):
     ]
 
 
+class _Variant(C.Structure):
+    _fields_ = [
+        ("model", C.c_void_p),
+        ("n_trajectories", C.c_uint64),
+        ("seed", C.c_uint64),
+        ("use_model_seed", C.c_int),
+        ("variant_id", C.c_uint32),
+    ]
+
+
+class _SweepInfo(C.Structure):
+    _fields_ = [("variants", C.c_int), ("classes", C.c_int), ("trajectories", C.c_uint64),
+                ("transitions", C.c_uint64), ("graph_ms", C.c_double), ("setup_ms", C.c_double)]
+
+
 _LIB = None
 
 
@@ -101,6 +116,14 @@ def lib():
         "mbg_get_final_states": (C.c_int, [vp, u64p]),
         "mbg_get_fixed_points": (C.c_int, [vp, u64p, u64p, sz, C.POINTER(sz)]),
         "mbg_get_summaries": (C.c_int, [vp, u64p, sz]),
+        "mbg_get_marginals": (C.c_int, [vp, f64p]),
+        "mbg_run_variants": (C.c_int, [vp, C.POINTER(_Variant), C.c_int]),
+        "mbg_get_sweep_info": (C.c_int, [vp, C.POINTER(_SweepInfo)]),
+        "mbg_get_variant_info": (C.c_int, [vp, C.c_int, C.POINTER(_RunInfo)]),
+        "mbg_get_variant_probtraj_raw": (C.c_int, [vp, C.c_int, u64p, u64p]),
+        "mbg_get_variant_marginals": (C.c_int, [vp, C.c_int, f64p]),
+        "mbg_get_variant_final_states": (C.c_int, [vp, C.c_int, u64p]),
+        "mbg_get_variant_fixed_points": (C.c_int, [vp, C.c_int, u64p, u64p, sz, C.POINTER(sz)]),
         "mbg_write_csv": (C.c_int, [vp, vp, cp]),
         "mbg_write_text": (C.c_int, [vp, vp, cp]),
     }
@@ -294,6 +317,60 @@ class Context:
                          i.n_trajectories, i.simulated, i.transitions, i.fixed_point_trajectories, i.kernel_ms,
                          i.time_tick, i.max_time, prob_raw.reshape(W, S), th_raw, prob.reshape(W, S), th, fin,
                          fs, fc, summ)
+
+    def marginals(self) -> np.ndarray:
+        """(W, K) probability that each readout is ON, per window, of the last run."""
+        i = self.info()
+        out = np.zeros(i.windows * i.readouts, np.float64)
+        self._check(lib().mbg_get_marginals(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(i.windows, i.readouts)
+
+    # ---- variant sweeps ----
+    def run_variants(self, variants):
+        """variants: iterable of (Model, n_trajectories, seed or None, variant_id).  All run in one
+        HIP graph launch; returns the sweep info (per-variant results through the getters below)."""
+        vs = list(variants)
+        arr = (_Variant * len(vs))()
+        self._sweep_models = [v[0] for v in vs]  # keep the models alive with the device results
+        for k, (model, n, seed, vid) in enumerate(vs):
+            arr[k] = _Variant(model.handle.value if hasattr(model.handle, "value") else model.handle, n,
+                              0 if seed is None else seed, int(seed is None), vid)
+        self._check(lib().mbg_run_variants(self._h, arr, len(vs)))
+        return self.sweep_info()
+
+    def sweep_info(self) -> _SweepInfo:
+        i = _SweepInfo()
+        self._check(lib().mbg_get_sweep_info(self._h, C.byref(i)))
+        return i
+
+    def variant_info(self, v) -> _RunInfo:
+        i = _RunInfo()
+        self._check(lib().mbg_get_variant_info(self._h, v, C.byref(i)))
+        return i
+
+    def variant_result(self, v):
+        """(info, prob_raw (W, 2^K), th_raw (W,), final (2^K,), fixed_states, fixed_counts)."""
+        L = lib()
+        i = self.variant_info(v)
+        W, S = i.windows, 1 << i.readouts
+        prob = np.zeros(W * S, np.uint64)
+        th = np.zeros(W, np.uint64)
+        self._check(L.mbg_get_variant_probtraj_raw(self._h, v, _u64p(prob), _u64p(th)))
+        fin = np.zeros(S, np.uint64)
+        self._check(L.mbg_get_variant_final_states(self._h, v, _u64p(fin)))
+        n = C.c_size_t()
+        self._check(L.mbg_get_variant_fixed_points(self._h, v, None, None, 0, C.byref(n)))
+        fs = np.zeros((n.value, i.state_words), np.uint64)
+        fc = np.zeros(n.value, np.uint64)
+        if n.value:
+            self._check(L.mbg_get_variant_fixed_points(self._h, v, _u64p(fs), _u64p(fc), n.value, C.byref(n)))
+        return i, prob.reshape(W, S), th, fin, fs, fc
+
+    def variant_marginals(self, v) -> np.ndarray:
+        i = self.variant_info(v)
+        out = np.zeros(i.windows * i.readouts, np.float64)
+        self._check(lib().mbg_get_variant_marginals(self._h, v, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(i.windows, i.readouts)
 
     def write_csv(self, model: Model, prefix: str):
         self._check(lib().mbg_write_csv(self._h, model.handle, prefix.encode()))

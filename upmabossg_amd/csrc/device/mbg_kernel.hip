@@ -12,6 +12,7 @@
 //   MBG_SCAN        candidates of the in-word flip search per round of table reads (2 or 4)
 //   MBG_WSUM_REG    1: the word rate sums live in VGPRs (more lanes fit in LDS), 0: in LDS
 //   MBG_RATE32      1: every fixed-point rate fits 32 bits, the rate table is u32
+//   MBG_MULTI       1: emit the variant-sweep kernel instead of the single-model one
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -277,22 +278,9 @@ __device__ __forceinline__ void flush_cell(const mbg_desc& D, int w, u32 s, u64 
 
 #define DEP_BATCH MBG_DEP_BATCH
 
-extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D) {
-    extern __shared__ u64 smem[];
-#if MBG_TABLES_LDS
-    // stage the model tables in LDS once per (persistent) block
-    {
-        const u32 words = D.lay.bytes / 8;
-        const u64* src = (const u64*)D.blob;
-        for (u32 i = threadIdx.x; i < words; i += MBG_BLOCK) smem[i] = src[i];
-        __syncthreads();
-    }
-    const char* tb = (const char*)smem;
-    u64* lane = smem + D.lay.bytes / 8;
-#else
-    const char* tb = D.blob;
-    u64* lane = smem;
-#endif
+// Simulates trajectories of one model (descriptor D) on this wave until its work queue is
+// empty and every lane of the wave is idle.
+__device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* tb, u64* lane) {
     const u32 tid = threadIdx.x;
     const u32 lid = tid & 63u;
     Lane L{D, tb, lane, tid};
@@ -658,5 +646,68 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
         hash = (hash ^ (u64)j) * 0x100000001b3ull;
 #endif
         k++;
+    }
+}
+
+#if !MBG_MULTI
+extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D) {
+    extern __shared__ u64 smem[];
+#if MBG_TABLES_LDS
+    // stage the model tables in LDS once per (persistent) block
+    {
+        const u32 words = D.lay.bytes / 8;
+        const u64* src = (const u64*)D.blob;
+        for (u32 i = threadIdx.x; i < words; i += MBG_BLOCK) smem[i] = src[i];
+        __syncthreads();
+    }
+    simulate_variant(D, (const char*)smem, smem + D.lay.bytes / 8);
+#else
+    simulate_variant(D, D.blob, smem);
+#endif
+}
+#else
+// Variant sweep: nv models of one state-word class in one persistent launch.  Each wave
+// drains one variant at a time (its descriptor is wave-uniform, read with scalar loads),
+// starting at a variant spread by wave index and moving on when that variant's queue is
+// empty; the variants' tables stay in HBM.
+extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const mbg_desc* descs, int nv) {
+    extern __shared__ u64 smem[];
+    const u32 waves_total = gridDim.x * (MBG_BLOCK / 64);
+    const u32 wave = blockIdx.x * (MBG_BLOCK / 64) + (threadIdx.x >> 6);
+    const int start = (int)(((u64)wave * (u64)nv) / waves_total);
+    for (int i = 0; i < nv; i++) {
+        int v = start + i;
+        if (v >= nv) v -= nv;
+        v = __builtin_amdgcn_readfirstlane(v);
+        const mbg_desc& D = descs[v];
+        simulate_variant(D, D.blob, smem);
+    }
+}
+#endif
+
+// ------------------------------------------------------------------ readout marginals
+
+// Marginal time each readout is ON per window, from a window table: out[w][j] = sum over the
+// projected states s with bit j set of prob[w][s] (exact 64-bit sums).  One block per window.
+extern "C" __global__ __launch_bounds__(256) void mbg_marginals(const u64* prob, int K, u64* out) {
+    __shared__ u64 part[256 * 20];
+    const u32 tid = threadIdx.x;
+    const u64 S = 1ull << K;
+    const u64* row = prob + ((u64)blockIdx.x << K);
+    u64 acc[20];
+#pragma unroll
+    for (int j = 0; j < 20; j++) acc[j] = 0;
+    for (u64 s = tid; s < S; s += 256) {
+        const u64 v = row[s];
+#pragma unroll
+        for (int j = 0; j < 20; j++) acc[j] += ((s >> j) & 1ull) ? v : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < 20; j++) part[j * 256 + tid] = acc[j];
+    __syncthreads();
+    if (tid < (u32)K) {
+        u64 sum = 0;
+        for (int i = 0; i < 256; i++) sum += part[tid * 256 + i];
+        out[(u64)blockIdx.x * K + tid] = sum;
     }
 }

@@ -11,9 +11,23 @@
 
 namespace mbg {
 
+// Host allocator for page-locked memory (fast device-to-host copies of large tables).
+template <class T>
+struct PinnedAllocator {
+    using value_type = T;
+    PinnedAllocator() = default;
+    template <class U>
+    PinnedAllocator(const PinnedAllocator<U>&) {}
+    T* allocate(size_t n);
+    void deallocate(T* p, size_t) noexcept;
+    bool operator==(const PinnedAllocator&) const { return true; }
+    bool operator!=(const PinnedAllocator&) const { return false; }
+};
+using PinnedU64 = std::vector<uint64_t, PinnedAllocator<uint64_t>>;
+
 struct Results {
     const mbg_run_info* info;
-    const std::vector<uint64_t>* prob;          // [W][2^K] fixed point (counts when discrete)
+    const PinnedU64* prob;                      // [W][2^K] fixed point (counts when discrete)
     const std::vector<uint64_t>* th;            // [W]
     const std::vector<uint64_t>* fin;           // [2^K]
     const std::vector<uint64_t>* fixed_states;  // [F][state_words], reference order

@@ -24,6 +24,20 @@
 #include "model.h"
 #include "output.h"
 
+namespace mbg {
+template <>
+uint64_t* PinnedAllocator<uint64_t>::allocate(size_t n) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, std::max<size_t>(1, n) * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
+        throw std::bad_alloc();
+    return static_cast<uint64_t*>(p);
+}
+template <>
+void PinnedAllocator<uint64_t>::deallocate(uint64_t* p, size_t) noexcept {
+    (void)hipHostFree(p);
+}
+}  // namespace mbg
+
 extern const char* const mbg_embedded_math;
 extern const char* const mbg_embedded_kernel;
 
@@ -78,12 +92,15 @@ struct Desc {
     int N, K, W, FP, FT, sigma;
 };
 
+using mbg_desc_ptr = const void*;
+
 struct Shape {
-    int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 4, scan = 4, wsum_reg = 0, rate32 = 0;
+    int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 4, scan = 4, wsum_reg = 0, rate32 = 0,
+        multi = 0;
     bool operator<(const Shape& o) const {
-        return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32) <
+        return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi) <
                std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in, o.dep_batch, o.scan,
-                        o.wsum_reg, o.rate32);
+                        o.wsum_reg, o.rate32, o.multi);
     }
 };
 
@@ -102,6 +119,7 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_SCAN " << sh.scan << "\n";
     os << "#define MBG_WSUM_REG " << sh.wsum_reg << "\n";
     os << "#define MBG_RATE32 " << sh.rate32 << "\n";
+    os << "#define MBG_MULTI " << sh.multi << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -119,7 +137,7 @@ bool rates_fit_u32(const mbg::RateTables& rt) {
     return true;
 }
 
-Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt) {
+Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt, bool rate32) {
     Blob b;
     std::vector<uint8_t> bytes;
     auto seg = [&](const void* p, size_t n) -> uint32_t {
@@ -132,7 +150,7 @@ Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt) {
     in16.resize(in16.size() + 32, 0);  // the kernel's unrolled gathers may read past the last node
     std::vector<uint32_t> ro(cm.readouts.begin(), cm.readouts.end());
     b.lay.nodes = seg(cm.nodes.data(), cm.nodes.size() * sizeof(mbg::NodeRec));
-    if (rates_fit_u32(rt)) {
+    if (rate32) {
         std::vector<uint32_t> r32(rt.rates.begin(), rt.rates.end());
         b.lay.rates = seg(r32.data(), r32.size() * 4);
     } else {
@@ -153,13 +171,14 @@ Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt) {
 
 struct Module {
     hipModule_t mod = nullptr;
-    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr;
+    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr;
 };
 
+// Device buffer kept by the context across runs; it only grows.
 template <class T>
 struct DevBuf {
     T* p = nullptr;
-    size_t n = 0;
+    size_t cap = 0;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
@@ -167,8 +186,13 @@ struct DevBuf {
         if (p) (void)hipFree(p);
     }
     void alloc(size_t count) {
-        n = count;
-        hip_check(hipMalloc((void**)&p, std::max<size_t>(1, count) * sizeof(T)), "hipMalloc");
+        count = std::max<size_t>(1, count);
+        if (p && count <= cap) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hip_check(hipMalloc((void**)&p, count * sizeof(T)), "hipMalloc");
+        cap = count;
     }
     void upload(const std::vector<T>& v, hipStream_t s) {
         alloc(v.size());
@@ -180,6 +204,39 @@ struct DevBuf {
     }
 };
 
+struct RunBuffers {
+    DevBuf<uint64_t> blob, rtab, stage, prob, th, fin, fp, fpc, summ, work, flips;
+    DevBuf<float> istate;
+};
+
+// One variant of a sweep: its tables, device buffers and kernel descriptor.
+struct VariantRun {
+    const mbg_model* m = nullptr;
+    int NWc = 0;        // words of its class (state rows are NWc words wide)
+    uint64_t n = 0;
+    RunBuffers buf;
+    DevBuf<uint64_t> marg;
+    Desc D{};
+    mbg_run_info info{};
+    std::vector<uint64_t> fixed_states, fixed_counts;
+    hipFunction_t marginals = nullptr;
+};
+
+struct SweepClass {
+    Shape sh{};
+    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr;
+    std::vector<int> members;
+    DevBuf<Desc> descs;
+    size_t smem = 0;
+    int blocks_per_cu = 1;
+};
+
+struct SweepState {
+    std::vector<std::unique_ptr<VariantRun>> vars;
+    std::vector<std::unique_ptr<SweepClass>> classes;
+    mbg_sweep_info info{};
+};
+
 }  // namespace
 
 struct mbg_ctx {
@@ -189,10 +246,14 @@ struct mbg_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::map<Shape, Module> modules;
+    RunBuffers buf;
+    std::unique_ptr<SweepState> sweep;       // last variant sweep
+    std::vector<hipStream_t> class_streams;  // one per word class of a sweep
     // last run
     bool have = false;
     mbg_run_info info{};
-    std::vector<uint64_t> prob, th, fin, fixed_states, fixed_counts, summ;
+    mbg::PinnedU64 prob;  // window table, pinned for a fast device-to-host copy
+    std::vector<uint64_t> th, fin, fixed_states, fixed_counts, summ;
 };
 
 namespace {
@@ -222,7 +283,9 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
     hiprtcDestroyProgram(&prog);
     Module m;
     hip_check(hipModuleLoadData(&m.mod, code.data()), "hipModuleLoadData");
-    hip_check(hipModuleGetFunction(&m.sim, m.mod, "mbg_simulate"), "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.sim, m.mod, sh.multi ? "mbg_simulate_multi" : "mbg_simulate"),
+              "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.marginals, m.mod, "mbg_marginals"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_state, m.mod, "mbg_init_state"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_rates, m.mod, "mbg_init_rates"), "hipModuleGetFunction");
     return c->modules.emplace(sh, m).first->second;
@@ -242,7 +305,8 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
                    int rate32) {
     const size_t cu_lds = 160 * 1024;
     const int max_in = std::max(1, cm.max_inputs);
-    const int wsum_reg = env_int("MBG_TUNE_WSUM_REG", 1) ? 1 : 0;
+    // word sums in VGPRs pay off while the unrolled select over the words stays short
+    const int wsum_reg = env_int("MBG_TUNE_WSUM_REG", cm.NW <= 16 ? 1 : 0) ? 1 : 0;
     const int tables_env = env_int("MBG_TUNE_TABLES", -1);
     if (!block_req) block_req = env_int("MBG_TUNE_BLOCK", 0);
     Shape best{cm.NW, 64, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, 0, max_in};
@@ -278,6 +342,54 @@ void launch(hipFunction_t f, uint64_t grid, int block, size_t smem, hipStream_t 
     hip_check(hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, (unsigned)smem, s, args, nullptr), what);
 }
 
+// Window count, fixed-point exponents and discrete step count of a run of n trajectories.
+struct RunScales {
+    int W = 0, FP = 0, FT = 0;
+    uint64_t S = 0, nsteps = 0;
+};
+
+RunScales run_scales(const mbg::CompiledModel& cm, uint64_t n) {
+    RunScales r;
+    r.W = (int)std::ceil(cm.max_time / cm.time_tick);  // window_average_small.cpp:26
+    r.S = 1ull << cm.K;
+    if (cm.discrete) {
+        r.FP = 0;
+        r.FT = 59 - mbg::ceil_log2_u64(n);
+    } else {
+        const int eq = mbg::ceil_log2_real((double)n * (double)cm.time_tick);
+        r.FP = 60 - eq;
+        r.FT = 57 - eq;
+    }
+    const float ratio = cm.max_time / cm.time_tick;
+    r.nsteps = ratio >= 2.f ? (uint64_t)std::floor(ratio) : 1;
+    return r;
+}
+
+// Distinct fixed points in the reference order (src/state.h:43-49): rows of `words` words,
+// of which the first `keep` are returned per state.
+void unique_fixed_points(const std::vector<uint64_t>& fps, int words, int keep, std::vector<uint64_t>& states,
+                         std::vector<uint64_t>& counts) {
+    const uint64_t nfp = fps.size() / (size_t)words;
+    std::vector<uint64_t> idx(nfp);
+    std::iota(idx.begin(), idx.end(), 0);
+    auto row = [&](uint64_t r) { return fps.data() + r * words; };
+    std::sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) {
+        const uint64_t *x = row(a), *y = row(b);
+        for (int w = words - 1; w >= 0; w--)
+            if (x[w] != y[w]) return x[w] < y[w];
+        return false;
+    });
+    states.clear();
+    counts.clear();
+    for (size_t i = 0; i < idx.size();) {
+        size_t j = i;
+        while (j < idx.size() && std::equal(row(idx[i]), row(idx[i]) + words, row(idx[j]))) j++;
+        states.insert(states.end(), row(idx[i]), row(idx[i]) + keep);
+        counts.push_back(j - i);
+        i = j;
+    }
+}
+
 void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     const mbg::CompiledModel& cm = m->cm;
     if (cm.K > mbg::kMaxReadouts) throw Error(MBG_ERR_LIMIT, "This executable supports a maximum of 20 non-internal nodes.");
@@ -291,8 +403,9 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     const mbg::RateTables rt = mbg::make_rate_tables(cm, m->mult_up.empty() ? nullptr : m->mult_up.data(),
                                                      m->mult_down.empty() ? nullptr : m->mult_down.data(),
                                                      m->istate.empty() ? nullptr : m->istate.data());
-    const Blob blob = pack_blob(cm, rt);
-    const Shape sh = choose_shape(c, cm, blob.lay.bytes, p->block, p->trace, rates_fit_u32(rt) ? 1 : 0);
+    const bool rate32 = rates_fit_u32(rt);
+    const Blob blob = pack_blob(cm, rt, rate32);
+    const Shape sh = choose_shape(c, cm, blob.lay.bytes, p->block, p->trace, rate32 ? 1 : 0);
     const Module& mod = get_module(c, cm, sh);
     const size_t smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0);
     int blocks_per_cu = 0;
@@ -319,8 +432,19 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     const uint64_t batch_max = std::max<uint64_t>(1, std::min<uint64_t>(count, (8ull << 30) / (24ull * (uint64_t)cm.NW)));
 
     hipStream_t s = c->stream;
-    DevBuf<uint64_t> d_blob, d_rtab, d_stage, d_prob, d_th, d_fin, d_fp, d_fpc, d_summ, d_work, d_flips;
-    DevBuf<float> d_istate;
+    RunBuffers& B = c->buf;
+    auto& d_blob = B.blob;
+    auto& d_rtab = B.rtab;
+    auto& d_stage = B.stage;
+    auto& d_prob = B.prob;
+    auto& d_th = B.th;
+    auto& d_fin = B.fin;
+    auto& d_fp = B.fp;
+    auto& d_fpc = B.fpc;
+    auto& d_summ = B.summ;
+    auto& d_work = B.work;
+    auto& d_flips = B.flips;
+    auto& d_istate = B.istate;
     d_blob.upload(blob.words, s);
     d_rtab.upload(rt.rtab, s);
     d_istate.upload(rt.istate, s);
@@ -448,6 +572,267 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     I.lds_bytes = (int)smem;
     I.blocks_per_cu = blocks_per_cu;
     c->have = true;
+}
+
+// Variant sweep: all variants in one HIP graph launch.
+void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
+    const auto t0 = std::chrono::steady_clock::now();
+    c->sweep.reset();
+    auto sw = std::make_unique<SweepState>();
+    hipStream_t main = c->stream;
+    std::map<std::pair<int, int>, int> class_of;  // (words, discrete) -> class
+    std::vector<mbg::RateTables> rts(nv);
+    std::vector<int> fits32(nv);
+    for (int v = 0; v < nv; v++) {
+        const mbg_model* m = vs[v].model;
+        if (!m) throw Error(MBG_ERR_ARG, "variant without a model");
+        const mbg::CompiledModel& cm = m->cm;
+        if (cm.K > mbg::kMaxReadouts) throw Error(MBG_ERR_LIMIT, "This executable supports a maximum of 20 non-internal nodes.");
+        if (!(cm.time_tick > 0.f) || !(cm.max_time > 0.f)) throw Error(MBG_ERR_ARG, "max_time and time_tick must be positive");
+        auto V = std::make_unique<VariantRun>();
+        V->m = m;
+        V->n = vs[v].n_trajectories ? vs[v].n_trajectories : (uint64_t)cm.sample_count;
+        if (V->n == 0 || V->n > (1ull << 32)) throw Error(MBG_ERR_ARG, "invalid trajectory count");
+        int nwc = 1;
+        while (nwc < cm.NW) nwc *= 2;
+        V->NWc = nwc;
+        rts[v] = mbg::make_rate_tables(cm, m->mult_up.empty() ? nullptr : m->mult_up.data(),
+                                       m->mult_down.empty() ? nullptr : m->mult_down.data(),
+                                       m->istate.empty() ? nullptr : m->istate.data());
+        fits32[v] = rates_fit_u32(rts[v]);
+        const auto key = std::make_pair(nwc, cm.discrete ? 1 : 0);
+        auto it = class_of.find(key);
+        if (it == class_of.end()) {
+            it = class_of.emplace(key, (int)sw->classes.size()).first;
+            sw->classes.push_back(std::make_unique<SweepClass>());
+            Shape& sh = sw->classes.back()->sh;
+            sh = Shape{nwc, 64, cm.discrete ? 1 : 0, 0, 0, 0, 1};
+            sh.rate32 = 1;
+            sh.multi = 1;
+            sh.wsum_reg = nwc <= 16 ? 1 : 0;
+        }
+        SweepClass& C = *sw->classes[it->second];
+        C.members.push_back(v);
+        C.sh.generic |= cm.any_generic ? 1 : 0;
+        C.sh.max_in = std::max(C.sh.max_in, cm.max_inputs);
+        C.sh.rate32 &= fits32[v];
+        sw->vars.push_back(std::move(V));
+    }
+    // per class: block size for the lanes' LDS, module, occupancy
+    const size_t cu_lds = 160 * 1024;
+    for (auto& Cp : sw->classes) {
+        SweepClass& C = *Cp;
+        int best_waves = 0;
+        for (int block = 1024; block >= 64; block -= 64) {
+            const size_t lds = lane_lds_bytes(C.sh.nw, block, C.sh.wsum_reg);
+            if (lds > (size_t)c->lds_per_block) continue;
+            const int waves = std::min(16, (int)(cu_lds / lds) * (block / 64));
+            if (waves > best_waves) {
+                best_waves = waves;
+                C.sh.block = block;
+            }
+        }
+        if (!best_waves) throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
+        C.sh.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 4)));
+        C.sh.scan = env_int("MBG_TUNE_SCAN", 4) == 2 ? 2 : 4;
+        const Module& mod = get_module(c, sw->vars[C.members[0]]->m->cm, C.sh);
+        C.sim = mod.sim;
+        C.init_state = mod.init_state;
+        C.init_rates = mod.init_rates;
+        C.marginals = mod.marginals;
+        C.smem = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg);
+        int nb = 0;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, C.sim, C.sh.block, C.smem) != hipSuccess || nb < 1) nb = 1;
+        C.blocks_per_cu = nb;
+    }
+    // per variant: tables and buffers
+    for (auto& Cp : sw->classes) {
+        SweepClass& C = *Cp;
+        std::vector<Desc> descs;
+        for (int v : C.members) {
+            VariantRun& V = *sw->vars[v];
+            const mbg::CompiledModel& cm = V.m->cm;
+            const mbg::RateTables& rt = rts[v];
+            const Blob blob = pack_blob(cm, rt, C.sh.rate32 != 0);
+            const RunScales rs = run_scales(cm, V.n);
+            const uint64_t fp_cap = V.n;
+            V.marginals = C.marginals;
+            RunBuffers& B = V.buf;
+            B.blob.upload(blob.words, main);
+            B.rtab.upload(rt.rtab, main);
+            B.istate.upload(rt.istate, main);
+            B.stage.alloc((size_t)V.n * 3 * V.NWc);
+            B.prob.zero((size_t)rs.W * rs.S, main);
+            B.th.zero((size_t)rs.W, main);
+            B.fin.zero((size_t)rs.S, main);
+            B.fp.alloc((size_t)fp_cap * V.NWc);
+            B.fpc.zero(1, main);
+            B.work.zero(1, main);
+            B.flips.zero(1, main);
+            Desc& D = V.D;
+            D = Desc{};
+            D.blob = B.blob.p;
+            D.rtab = B.rtab.p;
+            D.istate = B.istate.p;
+            D.stage = B.stage.p;
+            D.prob = B.prob.p;
+            D.th = B.th.p;
+            D.fin = B.fin.p;
+            D.fp_states = B.fp.p;
+            D.fp_count = B.fpc.p;
+            D.summ = nullptr;
+            D.work_counter = B.work.p;
+            D.flip_total = B.flips.p;
+            D.fp_cap = fp_cap;
+            D.summ_cap = 0;
+            D.nsteps = rs.nsteps;
+            D.batch = V.n;
+            D.batch_first = 0;
+            D.first_id = 0;
+            D.seed = vs[v].use_model_seed ? (uint64_t)cm.seed : vs[v].seed;
+            D.maxt = (double)cm.max_time;
+            D.tick = (double)cm.time_tick;
+            D.inv_tick = 1.0 / D.tick;
+            D.rscale = std::ldexp(1.0, -rt.E);
+            D.lay = blob.lay;
+            D.variant_id = vs[v].variant_id;
+            D.N = cm.N;
+            D.K = cm.K;
+            D.W = rs.W;
+            D.FP = rs.FP;
+            D.FT = rs.FT;
+            D.sigma = rt.sigma;
+            mbg_run_info& I = V.info;
+            I = mbg_run_info{};
+            I.windows = rs.W;
+            I.readouts = cm.K;
+            I.prob_exponent = rs.FP;
+            I.th_exponent = rs.FT;
+            I.rate_exponent = rt.E;
+            I.state_words = cm.NW;
+            I.discrete = cm.discrete ? 1 : 0;
+            I.n_trajectories = V.n;
+            I.simulated = V.n;
+            I.time_tick = cm.time_tick;
+            I.max_time = cm.max_time;
+            I.block = C.sh.block;
+            I.tables_in_lds = 0;
+            I.lds_bytes = (int)C.smem;
+            I.blocks_per_cu = C.blocks_per_cu;
+            descs.push_back(D);
+        }
+        C.descs.alloc(descs.size());
+        hip_check(hipMemcpyAsync(C.descs.p, descs.data(), descs.size() * sizeof(Desc), hipMemcpyHostToDevice, main),
+                  "hipMemcpyAsync");
+    }
+    hip_check(hipStreamSynchronize(main), "sweep set-up");
+    // one stream per class, forked from and joined to the main stream inside the graph
+    while (c->class_streams.size() < sw->classes.size()) {
+        hipStream_t st;
+        hip_check(hipStreamCreate(&st), "hipStreamCreate");
+        c->class_streams.push_back(st);
+    }
+    std::vector<hipEvent_t> evs(sw->classes.size() + 1);
+    for (auto& e : evs) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hip_check(hipStreamBeginCapture(main, hipStreamCaptureModeGlobal), "hipStreamBeginCapture");
+    hip_check(hipEventRecord(evs[0], main), "hipEventRecord");
+    for (size_t ci = 0; ci < sw->classes.size(); ci++) {
+        SweepClass& C = *sw->classes[ci];
+        hipStream_t st = c->class_streams[ci];
+        hip_check(hipStreamWaitEvent(st, evs[0], 0), "hipStreamWaitEvent");
+        for (int v : C.members) {
+            Desc& D = sw->vars[v]->D;
+            const uint64_t nwc = (uint64_t)sw->vars[v]->NWc;
+            const uint64_t init_threads = D.batch * nwc;
+            const uint64_t rate_blocks = nwc <= 128 ? (D.batch + 63) / 64 : (init_threads + 255) / 256;
+            launch(C.init_state, (init_threads + 255) / 256, 256, 0, st, D, "launch mbg_init_state");
+            launch(C.init_rates, rate_blocks, 256, 0, st, D, "launch mbg_init_rates");
+        }
+        uint64_t traj = 0;
+        for (int v : C.members) traj += sw->vars[v]->n;
+        uint64_t grid = (uint64_t)c->cus * (uint64_t)C.blocks_per_cu;
+        grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (traj + C.sh.block - 1) / C.sh.block));
+        const mbg_desc_ptr dp = C.descs.p;
+        int n_members = (int)C.members.size();
+        void* args[] = {(void*)&dp, &n_members};
+        hip_check(hipModuleLaunchKernel(C.sim, (unsigned)grid, 1, 1, (unsigned)C.sh.block, 1, 1, (unsigned)C.smem, st,
+                                        args, nullptr),
+                  "launch mbg_simulate_multi");
+        hip_check(hipEventRecord(evs[ci + 1], st), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(main, evs[ci + 1], 0), "hipStreamWaitEvent");
+    }
+    hip_check(hipStreamEndCapture(main, &graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    const auto t1 = std::chrono::steady_clock::now();
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    hip_check(hipEventRecord(e0, main), "hipEventRecord");
+    hip_check(hipGraphLaunch(exec, main), "hipGraphLaunch");
+    hip_check(hipEventRecord(e1, main), "hipEventRecord");
+    hip_check(hipStreamSynchronize(main), "variant sweep");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (auto& e : evs) (void)hipEventDestroy(e);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+
+    mbg_sweep_info& SI = sw->info;
+    SI = mbg_sweep_info{};
+    SI.variants = nv;
+    SI.classes = (int)sw->classes.size();
+    SI.graph_ms = ms;
+    SI.setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    for (auto& Vp : sw->vars) {
+        VariantRun& V = *Vp;
+        uint64_t nfp = 0, flips = 0;
+        hip_check(hipMemcpy(&nfp, V.buf.fpc.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        hip_check(hipMemcpy(&flips, V.buf.flips.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (nfp > V.D.fp_cap) throw Error(MBG_ERR_LIMIT, "fixed-point buffer overflow");
+        std::vector<uint64_t> fps((size_t)nfp * V.NWc);
+        if (nfp) hip_check(hipMemcpy(fps.data(), V.buf.fp.p, fps.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        unique_fixed_points(fps, V.NWc, V.m->cm.NW, V.fixed_states, V.fixed_counts);
+        V.info.transitions = flips;
+        V.info.fixed_point_trajectories = nfp;
+        V.info.kernel_ms = ms;
+        SI.trajectories += V.n;
+        SI.transitions += flips;
+    }
+    c->sweep = std::move(sw);
+}
+
+const VariantRun& sweep_variant(const mbg_ctx* c, int v) {
+    if (!c->sweep) throw Error(MBG_ERR_NOT_BUILT, "no variant sweep has run");
+    if (v < 0 || v >= (int)c->sweep->vars.size()) throw Error(MBG_ERR_ARG, "variant index out of range");
+    return *c->sweep->vars[v];
+}
+
+// Per-readout marginals of a window table held on the device.
+void device_marginals(mbg_ctx* c, const VariantRun& V, hipFunction_t fn, std::vector<uint64_t>& out) {
+    const int W = V.info.windows, K = V.info.readouts;
+    out.assign((size_t)W * K, 0);
+    if (K == 0) return;
+    DevBuf<uint64_t>& M = const_cast<DevBuf<uint64_t>&>(V.marg);
+    M.alloc((size_t)W * K);
+    const void* prob = V.buf.prob.p;
+    void* outp = M.p;
+    int k = K;
+    void* args[] = {(void*)&prob, &k, &outp};
+    hip_check(hipModuleLaunchKernel(fn, (unsigned)W, 1, 1, 256, 1, 1, 0, c->stream, args, nullptr), "launch mbg_marginals");
+    hip_check(hipMemcpyAsync(out.data(), M.p, out.size() * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "mbg_marginals");
+}
+
+void normalise_marginals(const mbg_run_info& I, const std::vector<uint64_t>& raw, double* out) {
+    for (size_t i = 0; i < raw.size(); i++) {
+        const double v = (double)raw[i];
+        out[i] = I.discrete ? v / (double)I.n_trajectories
+                            : std::ldexp(v, -I.prob_exponent) / ((double)I.n_trajectories * (double)I.time_tick);
+    }
 }
 
 mbg::Results results_view(const mbg_ctx* c) {
@@ -615,6 +1000,8 @@ void mbg_destroy(mbg_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     for (auto& kv : c->modules) (void)hipModuleUnload(kv.second.mod);
+    c->sweep.reset();
+    for (hipStream_t s : c->class_streams) (void)hipStreamDestroy(s);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -627,7 +1014,8 @@ mbg_status mbg_compile(mbg_ctx* c, const mbg_model* m, int trace) {
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         const mbg::CompiledModel& cm = m->cm;
         const mbg::RateTables rt = mbg::make_rate_tables(cm, nullptr, nullptr, nullptr);
-        get_module(c, cm, choose_shape(c, cm, pack_blob(cm, rt).lay.bytes, 0, trace, rates_fit_u32(rt) ? 1 : 0));
+        const bool r32 = rates_fit_u32(rt);
+        get_module(c, cm, choose_shape(c, cm, pack_blob(cm, rt, r32).lay.bytes, 0, trace, r32 ? 1 : 0));
     });
 }
 
@@ -715,6 +1103,92 @@ mbg_status mbg_write_text(const mbg_ctx* c, const mbg_model* m, const char* path
             if (!f) throw Error(MBG_ERR_ARG, std::string("cannot open ") + path);
             mbg::write_text(r, m->cm, f);
         }
+    });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+mbg_status mbg_get_marginals(const mbg_ctx* c, double* out) {
+    if (!c || !out) return MBG_ERR_ARG;
+    if (!c->have) return MBG_ERR_NOT_BUILT;
+    const int W = c->info.windows, K = c->info.readouts;
+    std::vector<uint64_t> raw((size_t)W * K, 0);
+    const uint64_t S = 1ull << K;
+    for (int w = 0; w < W; w++) {
+        const uint64_t* row = c->prob.data() + (uint64_t)w * S;
+        uint64_t* o = raw.data() + (size_t)w * K;
+        for (uint64_t st = 0; st < S; st++) {
+            const uint64_t v = row[st];
+            if (!v) continue;
+            for (int j = 0; j < K; j++)
+                if ((st >> j) & 1ull) o[j] += v;
+        }
+    }
+    normalise_marginals(c->info, raw, out);
+    return MBG_OK;
+}
+
+mbg_status mbg_run_variants(mbg_ctx* c, const mbg_variant* variants, int count) {
+    if (!c || !variants || count <= 0) return MBG_ERR_ARG;
+    return guarded(c, [&] {
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        run_variants_impl(c, variants, count);
+    });
+}
+
+mbg_status mbg_get_sweep_info(const mbg_ctx* c, mbg_sweep_info* out) {
+    if (!c || !out) return MBG_ERR_ARG;
+    if (!c->sweep) return MBG_ERR_NOT_BUILT;
+    *out = c->sweep->info;
+    return MBG_OK;
+}
+
+mbg_status mbg_get_variant_info(const mbg_ctx* c, int v, mbg_run_info* out) {
+    if (!c || !out) return MBG_ERR_ARG;
+    return guarded(const_cast<mbg_ctx*>(c), [&] { *out = sweep_variant(c, v).info; });
+}
+
+mbg_status mbg_get_variant_probtraj_raw(const mbg_ctx* c, int v, uint64_t* prob, uint64_t* th) {
+    if (!c) return MBG_ERR_ARG;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        const VariantRun& V = sweep_variant(c, v);
+        const size_t W = (size_t)V.info.windows, S = (size_t)1 << V.info.readouts;
+        if (prob) hip_check(hipMemcpy(prob, V.buf.prob.p, W * S * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (th) hip_check(hipMemcpy(th, V.buf.th.p, W * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+mbg_status mbg_get_variant_marginals(const mbg_ctx* c, int v, double* out) {
+    if (!c || !out) return MBG_ERR_ARG;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        const VariantRun& V = sweep_variant(c, v);
+        std::vector<uint64_t> raw;
+        device_marginals(const_cast<mbg_ctx*>(c), V, V.marginals, raw);
+        normalise_marginals(V.info, raw, out);
+    });
+}
+
+mbg_status mbg_get_variant_final_states(const mbg_ctx* c, int v, uint64_t* counts) {
+    if (!c || !counts) return MBG_ERR_ARG;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        const VariantRun& V = sweep_variant(c, v);
+        const size_t S = (size_t)1 << V.info.readouts;
+        hip_check(hipMemcpy(counts, V.buf.fin.p, S * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+mbg_status mbg_get_variant_fixed_points(const mbg_ctx* c, int v, uint64_t* states, uint64_t* counts, size_t cap,
+                                        size_t* n) {
+    if (!c) return MBG_ERR_ARG;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        const VariantRun& V = sweep_variant(c, v);
+        const size_t F = V.fixed_counts.size(), NW = (size_t)V.info.state_words;
+        if (n) *n = F;
+        const size_t k = std::min(cap, F);
+        if (states && k) std::memcpy(states, V.fixed_states.data(), k * NW * 8);
+        if (counts && k) std::memcpy(counts, V.fixed_counts.data(), k * 8);
     });
 }
 
