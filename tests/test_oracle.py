@@ -40,7 +40,7 @@ def test_log_accuracy():
             assert got == 0.0
             continue
         worst = max(worst, abs(got - ref) / (abs(ref) * 2.220446049250313e-16))
-    assert worst <= 2.0
+    assert worst <= 5.0
     # -ln((v + 1) 2^-32) endpoints
     assert L.orc_neglog_u32(0xFFFFFFFF) == 0.0
     assert abs(L.orc_neglog_u32(0) - 32 * math.log(2)) < 1e-14

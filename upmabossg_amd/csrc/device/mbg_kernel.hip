@@ -9,10 +9,10 @@
 //   MBG_TABLES_LDS  1 to stage the model tables in LDS (they fit), 0 to read them from HBM
 //   MBG_MAX_IN      largest number of inputs of a node (gathers are unrolled up to 6)
 //   MBG_DEP_BATCH   dependents evaluated per round of independent table reads
-//   MBG_SCAN        candidates of the in-word flip search per round of table reads (2 or 4)
 //   MBG_WSUM_REG    1: the word rate sums live in VGPRs (more lanes fit in LDS), 0: in LDS
 //   MBG_RATE32      1: every fixed-point rate fits 32 bits, the rate table is u32
 //   MBG_MULTI       1: emit the variant-sweep kernel instead of the single-model one
+//   MBG_SUMW        64-bit words per rate-sum group (register sums only; divides MBG_NW)
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -111,31 +111,29 @@ struct mbg_desc {
 #define LANE_X(w) lane[(0 * MBG_NW + (w)) * MBG_BLOCK + tid]
 #define LANE_A(w) lane[(1 * MBG_NW + (w)) * MBG_BLOCK + tid]
 #if MBG_WSUM_REG
-// word sums in registers: every access is an unrolled select over the NW words
-#define LANE_S(w) ws[w]
+// rate sums of groups of MBG_SUMW words in registers: every access is an unrolled select
+// over the MBG_NW / MBG_SUMW groups
+#define NSUM (MBG_NW / MBG_SUMW)
+#define LANE_S(g) ws[g]
 #define WS_ADD(w, d)                                                       \
     do {                                                                   \
-        const u32 w_ = (w);                                                \
+        const u32 g_ = (w) / MBG_SUMW;                                     \
         const u64 d_ = (d);                                                \
-        _Pragma("unroll") for (int i_ = 0; i_ < MBG_NW; i_++) ws[i_] += w_ == (u32)i_ ? d_ : 0ull; \
+        _Pragma("unroll") for (int i_ = 0; i_ < NSUM; i_++) ws[i_] += g_ == (u32)i_ ? d_ : 0ull; \
     } while (0)
-#define LANE_WORDS 2
 #else
+#undef MBG_SUMW
+#define MBG_SUMW 1
+#define NSUM MBG_NW
 #define LANE_S(w) lane[(2 * MBG_NW + (w)) * MBG_BLOCK + tid]
 #define WS_ADD(w, d) LANE_S(w) = LANE_S(w) + (d)
-#define LANE_WORDS 3
 #endif
 
 // ------------------------------------------------------------------ pre-pass kernels
 
-// Pass 1, one thread per (trajectory, word): initial state bits.  Node i is ON iff the
-// i-th uniform of the trajectory's stream 0 is <= istate_i (simulation.cu:34-71).
-extern "C" __global__ __launch_bounds__(256) void mbg_init_state(mbg_desc D) {
-    const u64 gid = (u64)blockIdx.x * 256u + threadIdx.x;
-    if (gid >= D.batch * MBG_NW) return;
-    const u64 t = gid / MBG_NW;
-    const int w = (int)(gid - t * MBG_NW);
-    const u32 id = (u32)(D.batch_first + t);
+// Initial state word w of trajectory id: node i is ON iff the i-th uniform of the
+// trajectory's stream 0 is <= istate_i (simulation.cu:34-71).
+__device__ __forceinline__ u64 initial_word(const mbg_desc& D, u32 id, int w) {
     u64 word = 0;
     for (int q = 0; q < 16; q++) {
         const int i0 = w * 64 + q * 4;
@@ -148,7 +146,16 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_state(mbg_desc D) {
             if (i < D.N && mbg_unit_f32(o[r]) <= D.istate[i]) word |= 1ull << (i & 63);
         }
     }
-    D.stage[t * 3 * MBG_NW + w] = word;
+    return word;
+}
+
+// Pass 1 (models above 128 words), one thread per (trajectory, word): initial state bits.
+extern "C" __global__ __launch_bounds__(256) void mbg_init_state(mbg_desc D) {
+    const u64 gid = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (gid >= D.batch * MBG_NW) return;
+    const u64 t = gid / MBG_NW;
+    const int w = (int)(gid - t * MBG_NW);
+    D.stage[t * 3 * MBG_NW + w] = initial_word(D, (u32)(D.batch_first + t), w);
 }
 
 __device__ __forceinline__ u64 rate_from_key(const mbg_desc& D, const char* tb, const mbg_node& n, u32 i, u32 key) {
@@ -161,11 +168,11 @@ __device__ __forceinline__ u64 rate_from_key(const mbg_desc& D, const char* tb, 
     return xi == logic ? 0ull : TRATE(2 * i + xi);
 }
 
-// Pass 2: active set and word rate sums.  A block takes 64 trajectories: their state
-// words are staged in LDS (lane-interleaved), then each wave sweeps whole words for the
-// 64 trajectories, one per lane, so the table reads of a word's 64 nodes are uniform
-// across the wave and the state reads never conflict.  Models above 128 words use the
-// per-(trajectory, word) variant below.
+// Initial states, active sets and word rate sums.  A block takes 64 trajectories: it draws
+// their state words into LDS (lane-interleaved) and the staging area, then each wave sweeps
+// whole words for the 64 trajectories, one per lane, so the table reads of a word's 64
+// nodes are uniform across the wave and the state reads never conflict.  Models above 128
+// words run mbg_init_state first and use the per-(trajectory, word) variant below.
 #define INIT_TRAJ 64
 #if MBG_NW <= 128
 extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
@@ -175,7 +182,12 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
     const u64 t0 = (u64)blockIdx.x * INIT_TRAJ;
     for (u32 idx = tid; idx < MBG_NW * INIT_TRAJ; idx += 256) {
         const u32 w = idx / INIT_TRAJ, l = idx % INIT_TRAJ;
-        xs[idx] = t0 + l < D.batch ? D.stage[(t0 + l) * 3 * MBG_NW + w] : 0ull;
+        u64 word = 0;
+        if (t0 + l < D.batch) {
+            word = initial_word(D, (u32)(D.batch_first + t0 + l), (int)w);
+            D.stage[(t0 + l) * 3 * MBG_NW + w] = word;
+        }
+        xs[idx] = word;
     }
     __syncthreads();
     const u64 t = t0 + lane;
@@ -290,9 +302,9 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
     u64 id = 0, total = 0, k = 0, acc_p = 0, acc_th = 0, flips = 0;
     u64 ro_R = 0, ro_S = 0;  // readout rate sum and entropy-term sum (exact)
 #if MBG_WSUM_REG
-    u64 ws[MBG_NW];
+    u64 ws[NSUM];
 #pragma unroll
-    for (int w = 0; w < MBG_NW; w++) ws[w] = 0;
+    for (int g = 0; g < NSUM; g++) ws[g] = 0;
 #endif
     int ro_act = 0;          // readouts with a non-zero rate
     double t = 0.0, th = 0.0;
@@ -321,12 +333,17 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     const u64* st = D.stage + g * 3 * MBG_NW;
                     total = 0;
 #pragma unroll
-                    for (int w = 0; w < MBG_NW; w++) {
-                        const u64 xw = st[w], aw = st[MBG_NW + w], sw = st[2 * MBG_NW + w];
-                        LANE_X(w) = xw;
-                        LANE_A(w) = aw;
-                        LANE_S(w) = sw;
-                        total += sw;
+                    for (int g = 0; g < NSUM; g++) {
+                        u64 gs = 0;
+#pragma unroll
+                        for (int q = 0; q < MBG_SUMW; q++) {
+                            const int w = g * MBG_SUMW + q;
+                            LANE_X(w) = st[w];
+                            LANE_A(w) = st[MBG_NW + w];
+                            gs += st[2 * MBG_NW + w];
+                        }
+                        LANE_S(g) = gs;
+                        total += gs;
                     }
                     s = 0;
                     ro_R = 0;
@@ -477,101 +494,70 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             continue;
         }
 
-        // ---- flip selection: word sums, then the active bits of one word ----
+        // ---- flip selection: group sums, then the active bits of the group's words ----
         const u64 target = mbg_flip_target(dr2, total);
         u64 acc = 0;
-        int wsel = MBG_NW - 1;
+        int gsel = NSUM - 1;
         bool found = false;
 #pragma unroll
-        for (int w = 0; w < MBG_NW; w++) {
-            const u64 sw = LANE_S(w);
+        for (int g = 0; g < NSUM; g++) {
+            const u64 sg = LANE_S(g);
             if (!found) {
-                if (acc + sw > target) {
-                    wsel = w;
+                if (acc + sg > target) {
+                    gsel = g;
                     found = true;
                 } else {
-                    acc += sw;
+                    acc += sg;
                 }
             }
         }
         u64 rem = target - acc;
-        u64 m = LANE_A(wsel);
-        const u64 xw = LANE_X(wsel);
+        // guard for an inconsistent group (never taken when the sums are exact)
+        u32 j = (u32)(gsel * MBG_SUMW * 64);
+        u64 r_sel = 0, xw = LANE_X(j >> 6);
+        bool hit = false;
+        for (int q = 0; q < MBG_SUMW && !hit; q++) {
+            const u32 wi = (u32)(gsel * MBG_SUMW + q);
+            u64 m = LANE_A(wi);
+            const u64 xq = LANE_X(wi);
 #if MBG_GENERIC
-        const u64 gm = TGEN(wsel);
+            const u64 gm = TGEN(wi);
 #endif
-        const u32 wbase = (u32)wsel * 64u;
-        // guard for an inconsistent word (never taken when the sums are exact): last active node
-        u32 j = m ? (u32)(wsel * 64 + 63 - __clzll((long long)m)) : wbase;
-        u64 r_sel = 0;
-#if MBG_SCAN == 4
-        while (m) {
-            // four candidates per round trip to the rate table
-            u32 bb[4];
-            u64 rr[4], mm = m;
+            const u32 wbase = wi * 64u;
+            while (m) {
+                // four candidates per round trip to the rate table
+                u32 bb[4];
+                u64 rr[4], mm = m;
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                bb[c] = mm ? (u32)__builtin_ctzll(mm) : 64u;
-                mm = mm ? (mm & (mm - 1ull)) : 0ull;
-                const u32 b = bb[c] < 64u ? bb[c] : 0u;
-                rr[c] = bb[c] < 64u ? TRATE(2 * (wbase + b) + ((xw >> b) & 1ull)) : 0ull;
+                for (int c = 0; c < 4; c++) {
+                    bb[c] = mm ? (u32)__builtin_ctzll(mm) : 64u;
+                    mm = mm ? (mm & (mm - 1ull)) : 0ull;
+                    const u32 b = bb[c] < 64u ? bb[c] : 0u;
+                    rr[c] = bb[c] < 64u ? TRATE(2 * (wbase + b) + ((xq >> b) & 1ull)) : 0ull;
 #if MBG_GENERIC
-                if (bb[c] < 64u && ((gm >> b) & 1ull)) {
-                    const mbg_node nd = TNODE(wbase + b);
-                    rr[c] = rate_from_key(D, tb, nd, wbase + b, L.key_of(nd, wbase + b));
+                    if (bb[c] < 64u && ((gm >> b) & 1ull)) {
+                        const mbg_node nd = TNODE(wbase + b);
+                        rr[c] = rate_from_key(D, tb, nd, wbase + b, L.key_of(nd, wbase + b));
+                    }
+#endif
                 }
-#endif
-            }
-            bool hit = false;
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                if (!hit && bb[c] < 64u) {
-                    if (rem < rr[c]) {
-                        j = wbase + bb[c];
-                        r_sel = rr[c];
-                        hit = true;
-                    } else {
-                        rem -= rr[c];
+                for (int c = 0; c < 4; c++) {
+                    if (!hit && bb[c] < 64u) {
+                        if (rem < rr[c]) {
+                            j = wbase + bb[c];
+                            r_sel = rr[c];
+                            xw = xq;
+                            hit = true;
+                        } else {
+                            rem -= rr[c];
+                        }
                     }
                 }
+                if (hit) break;
+                m = mm;
             }
-            if (hit) break;
-            m = mm;
         }
-#else
-        while (m) {
-            // two candidates per round trip to the rate table
-            const u32 b0 = (u32)__builtin_ctzll(m);
-            const u64 m1 = m & (m - 1ull);
-            const u32 b1 = m1 ? (u32)__builtin_ctzll(m1) : b0;
-            u64 r0 = TRATE(2 * (wbase + b0) + ((xw >> b0) & 1ull));
-            u64 r1 = TRATE(2 * (wbase + b1) + ((xw >> b1) & 1ull));
-#if MBG_GENERIC
-            if ((gm >> b0) & 1ull) {
-                const mbg_node nd = TNODE(wbase + b0);
-                r0 = rate_from_key(D, tb, nd, wbase + b0, L.key_of(nd, wbase + b0));
-            }
-            if ((gm >> b1) & 1ull) {
-                const mbg_node nd = TNODE(wbase + b1);
-                r1 = rate_from_key(D, tb, nd, wbase + b1, L.key_of(nd, wbase + b1));
-            }
-#endif
-            if (rem < r0) {
-                j = wbase + b0;
-                r_sel = r0;
-                break;
-            }
-            rem -= r0;
-            if (!m1) break;
-            if (rem < r1) {
-                j = wbase + b1;
-                r_sel = r1;
-                break;
-            }
-            rem -= r1;
-            m = m1 & (m1 - 1ull);
-        }
-#endif
 
         // ---- flip j and update the exact sums ----
         const u32 xj_old = (u32)(xw >> (j & 63)) & 1u;

@@ -9,8 +9,9 @@
 //    replaces the reference's per-slot curand XORWOW state
 //    (src/jit_kernels/simulation.cu:23-32).
 //  * mbg_ln: fdlibm-style argument reduction to [sqrt(1/2), sqrt(2)) and the
-//    atanh series ln m = 2g(1 + g^2/3 + g^4/5 + ... + g^22/23), g = (m-1)/(m+1).
-//    Max error about 2 ulp.  Used for the Gillespie waiting time and entropies.
+//    atanh series ln m = 2g(1 + g^2/3 + g^4/5 + ... + g^16/17), g = (m-1)/(m+1),
+//    |g| <= 0.1716: truncation below 1e-15 relative, max error about 5 ulp.  Used for the
+//    Gillespie waiting time and entropies.
 #pragma once
 
 #ifndef MBG_HD
@@ -58,10 +59,7 @@ MBG_HD double mbg_ln(double x) {
     const double s = m + 1.0;
     const double g = f / s;
     const double z = g * g;
-    double p = 1.0 / 23.0;
-    p = p * z + 1.0 / 21.0;
-    p = p * z + 1.0 / 19.0;
-    p = p * z + 1.0 / 17.0;
+    double p = 1.0 / 17.0;
     p = p * z + 1.0 / 15.0;
     p = p * z + 1.0 / 13.0;
     p = p * z + 1.0 / 11.0;

@@ -96,11 +96,11 @@ using mbg_desc_ptr = const void*;
 
 struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 4, scan = 4, wsum_reg = 0, rate32 = 0,
-        multi = 0;
+        multi = 0, sumw = 1;
     bool operator<(const Shape& o) const {
-        return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi) <
-               std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in, o.dep_batch, o.scan,
-                        o.wsum_reg, o.rate32, o.multi);
+        return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
+                        sumw) < std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in,
+                                         o.dep_batch, o.scan, o.wsum_reg, o.rate32, o.multi, o.sumw);
     }
 };
 
@@ -116,10 +116,10 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_TABLES_LDS " << sh.tables_lds << "\n";
     os << "#define MBG_MAX_IN " << sh.max_in << "\n";
     os << "#define MBG_DEP_BATCH " << sh.dep_batch << "\n";
-    os << "#define MBG_SCAN " << sh.scan << "\n";
     os << "#define MBG_WSUM_REG " << sh.wsum_reg << "\n";
     os << "#define MBG_RATE32 " << sh.rate32 << "\n";
     os << "#define MBG_MULTI " << sh.multi << "\n";
+    os << "#define MBG_SUMW " << sh.sumw << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -298,6 +298,16 @@ int env_int(const char* name, int dflt) {
     return e ? std::atoi(e) : dflt;
 }
 
+// Words per register rate-sum group: fewer, wider groups make each sum update cheaper (an
+// unrolled select over the groups) at the cost of scanning more act words per selection.
+int default_sumw(int nw, int wsum_reg) {
+    if (!wsum_reg) return 1;
+    int sw = env_int("MBG_TUNE_SUMW", 1);
+    if (sw < 1) sw = 1;
+    while (sw > 1 && nw % sw) sw /= 2;
+    return sw;
+}
+
 // Launch shape: block size, table placement (LDS or HBM) and word-sum placement (LDS or
 // VGPRs).  Without tuning knobs: stage the tables in LDS when they fit beside the lanes'
 // state and keep the most waves resident per CU (160 KiB of LDS per CU on gfx950).
@@ -329,7 +339,7 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
         }
     }
     best.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 4)));
-    best.scan = env_int("MBG_TUNE_SCAN", 4) == 2 ? 2 : 4;
+    best.sumw = default_sumw(cm.NW, best.wsum_reg);
     if (best_score < 0) {
         if (block_req) throw Error(MBG_ERR_LIMIT, "the requested block does not fit in LDS for this model");
         throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
@@ -501,7 +511,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
         const uint64_t init_threads = D.batch * (uint64_t)cm.NW;
         // mbg_init_rates: one block per 64 trajectories (per (trajectory, word) above 128 words)
         const uint64_t rate_blocks = cm.NW <= 128 ? (D.batch + 63) / 64 : (init_threads + 255) / 256;
-        launch(mod.init_state, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_state");
+        if (cm.NW > 128) launch(mod.init_state, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_state");
         launch(mod.init_rates, rate_blocks, 256, 0, s, D, "launch mbg_init_rates");
         uint64_t grid = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)blocks_per_cu;
         grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (D.batch + sh.block - 1) / sh.block));
@@ -610,6 +620,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             sh.rate32 = 1;
             sh.multi = 1;
             sh.wsum_reg = nwc <= 16 ? 1 : 0;
+            sh.sumw = default_sumw(nwc, sh.wsum_reg);
         }
         SweepClass& C = *sw->classes[it->second];
         C.members.push_back(v);
@@ -634,7 +645,6 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         }
         if (!best_waves) throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
         C.sh.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 4)));
-        C.sh.scan = env_int("MBG_TUNE_SCAN", 4) == 2 ? 2 : 4;
         const Module& mod = get_module(c, sw->vars[C.members[0]]->m->cm, C.sh);
         C.sim = mod.sim;
         C.init_state = mod.init_state;
@@ -747,7 +757,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             const uint64_t nwc = (uint64_t)sw->vars[v]->NWc;
             const uint64_t init_threads = D.batch * nwc;
             const uint64_t rate_blocks = nwc <= 128 ? (D.batch + 63) / 64 : (init_threads + 255) / 256;
-            launch(C.init_state, (init_threads + 255) / 256, 256, 0, st, D, "launch mbg_init_state");
+            if (nwc > 128) launch(C.init_state, (init_threads + 255) / 256, 256, 0, st, D, "launch mbg_init_state");
             launch(C.init_rates, rate_blocks, 256, 0, st, D, "launch mbg_init_rates");
         }
         uint64_t traj = 0;
