@@ -1,0 +1,56 @@
+/*
+ * mbg_mlp.h - C ABI of libmbg_mlp.so: the sweep surrogate, a multilayer perceptron trained
+ * and evaluated on an MI355X (gfx950) with hand-written BF16 MFMA kernels (FP32 accumulation).
+ *
+ * No reference counterpart: MaBoSSG has no learned component.  The surrogate maps a model
+ * variant's parameters (log rate multipliers and initial probabilities) to the windowed
+ * readout marginals the simulator produces; the simulator generates its training data and
+ * judges its error.
+ *
+ * Network: n_in -> hidden -> ... -> hidden -> n_out, GELU (tanh form) after every hidden layer,
+ * sigmoid on the output, mean squared error loss, Adam.  Inputs are padded to a multiple of
+ * 64 columns with zeros; weights are stored [out][in].
+ */
+#ifndef MBG_MLP_H
+#define MBG_MLP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int mbg_status; /* same codes as mbg.h: 0 ok, 3 invalid argument, 4 HIP error */
+
+typedef struct mbg_mlp mbg_mlp;
+
+/* hidden_layers hidden layers of width hidden (a multiple of 64); weights drawn from the
+ * uniform Glorot range with the given seed. */
+mbg_status mbg_mlp_create(int device, int n_in, int hidden, int hidden_layers, int n_out, int max_batch, uint64_t seed,
+                          mbg_mlp** out);
+void mbg_mlp_destroy(mbg_mlp* m);
+const char* mbg_mlp_last_error(const mbg_mlp* m);
+
+/* Parameters as FP32, layer by layer: W [out][in_padded] then b [out]. */
+size_t mbg_mlp_num_params(const mbg_mlp* m);
+int mbg_mlp_padded_inputs(const mbg_mlp* m);
+mbg_status mbg_mlp_get_params(mbg_mlp* m, float* out);
+mbg_status mbg_mlp_set_params(mbg_mlp* m, const float* in);
+
+/* Loss and gradients (same layout as the parameters) of one batch, without updating. */
+mbg_status mbg_mlp_gradients(mbg_mlp* m, const float* x, const float* y, int batch, float* loss, float* grads);
+
+/* Training set held on the device: x [n][n_in], y [n][n_out]. */
+mbg_status mbg_mlp_load_dataset(mbg_mlp* m, const float* x, const float* y, size_t n);
+/* epochs over the device training set in shuffled mini-batches of `batch` rows (a multiple of 64);
+ * Adam with learning rate lr (beta1 0.9, beta2 0.999, eps 1e-8); mean loss of the last epoch. */
+mbg_status mbg_mlp_train(mbg_mlp* m, int epochs, int batch, float lr, uint64_t shuffle_seed, float* last_loss);
+/* Predictions y [n][n_out] for x [n][n_in] (host buffers); kernel time in ms. */
+mbg_status mbg_mlp_predict(mbg_mlp* m, const float* x, size_t n, float* y, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MBG_MLP_H */

@@ -7,6 +7,7 @@
   compiled at run time by hipRTC for gfx950) and links the CLI against it.
 * Compile-checks the generated kernel for gfx950 with hipcc for two shapes, so a
   device-code error shows up at build time rather than at the first run.
+* Compiles the surrogate MLP library (libmbg_mlp.so, BF16 MFMA kernels) for gfx950.
 * Builds the CPU oracle (oracle/liboracle.so, test infrastructure).
 """
 from __future__ import annotations
@@ -132,6 +133,10 @@ def build(verbose=False):
         lib = os.path.join(PKG, "libmbg.so")
         _run([CXX, "-shared", "-fPIC", "-o", lib + ".tmp", *objs, *HOST_LIBS], verbose)
         os.replace(lib + ".tmp", lib)
+        mlp = os.path.join(PKG, "libmbg_mlp.so")
+        _run([HIPCC, *KERNEL_FLAGS, "-fPIC", "-shared", "-o", mlp + ".tmp", os.path.join(CSRC, "mlp", "mbg_mlp.hip")],
+             verbose)
+        os.replace(mlp + ".tmp", mlp)
         cli = os.path.join(PKG, "MaBoSSG")
         _run([CXX, *HOST_FLAGS, "-o", cli + ".tmp", os.path.join(CSRC, "cli.cpp"), "-L" + PKG, "-lmbg",
               "-Wl,-rpath,$ORIGIN", *HOST_LIBS], verbose)
