@@ -42,6 +42,9 @@ mbg_status mbg_model_parse(const char* bnd_text, const char* cfg_text, mbg_model
 mbg_status mbg_model_parse_files(const char* bnd_path, const char* cfg_path, mbg_model** out, char* err,
                                  size_t errlen);
 void mbg_model_free(mbg_model* m);
+/* A copy of a model (sharing its parsed form) with the same variant overrides; cheap, for
+ * building many variants of one network. */
+mbg_status mbg_model_clone(const mbg_model* m, mbg_model** out);
 
 /* Model facts (driver::nodes / constants, src/parser/driver.h:41-43). */
 int mbg_model_num_nodes(const mbg_model* m);

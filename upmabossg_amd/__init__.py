@@ -96,6 +96,7 @@ def lib():
         "mbg_model_parse": (C.c_int, [cp, cp, C.POINTER(vp), cp, sz]),
         "mbg_model_parse_files": (C.c_int, [cp, cp, C.POINTER(vp), cp, sz]),
         "mbg_model_free": (None, [vp]),
+        "mbg_model_clone": (C.c_int, [vp, C.POINTER(vp)]),
         "mbg_model_num_nodes": (C.c_int, [vp]),
         "mbg_model_node_name": (cp, [vp, C.c_int]),
         "mbg_model_node_istate": (C.c_float, [vp, C.c_int]),
@@ -150,6 +151,16 @@ class Model:
         if st != MBG_OK:
             raise MbgError(st, err.value.decode())
         self._h = h
+
+    def clone(self) -> "Model":
+        """A copy sharing the parsed network (cheap), with the same variant overrides."""
+        h = C.c_void_p()
+        st = lib().mbg_model_clone(self._h, C.byref(h))
+        if st != MBG_OK:
+            raise MbgError(st, "clone failed")
+        m = Model.__new__(Model)
+        m._h = h
+        return m
 
     @classmethod
     def from_files(cls, bnd_path: str, cfg_path: str) -> "Model":

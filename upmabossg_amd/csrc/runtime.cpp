@@ -42,7 +42,7 @@ extern const char* const mbg_embedded_math;
 extern const char* const mbg_embedded_kernel;
 
 struct mbg_model {
-    mbg::ParsedModel parsed;
+    std::shared_ptr<mbg::ParsedModel> parsed;  // shared by clones
     mbg::CompiledModel cm;
     std::vector<float> mult_up, mult_down, istate;  // empty: model's own values
 };
@@ -248,6 +248,7 @@ struct mbg_ctx {
     std::map<Shape, Module> modules;
     RunBuffers buf;
     std::unique_ptr<SweepState> sweep;       // last variant sweep
+    std::vector<std::unique_ptr<VariantRun>> variant_pool;  // device buffers reused by later sweeps
     std::vector<hipStream_t> class_streams;  // one per word class of a sweep
     // last run
     bool have = false;
@@ -587,7 +588,10 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
 // Variant sweep: all variants in one HIP graph launch.
 void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
     const auto t0 = std::chrono::steady_clock::now();
-    c->sweep.reset();
+    if (c->sweep) {  // return the previous sweep's buffers to the pool
+        for (auto& V : c->sweep->vars) c->variant_pool.push_back(std::move(V));
+        c->sweep.reset();
+    }
     auto sw = std::make_unique<SweepState>();
     hipStream_t main = c->stream;
     std::map<std::pair<int, int>, int> class_of;  // (words, discrete) -> class
@@ -599,7 +603,15 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         const mbg::CompiledModel& cm = m->cm;
         if (cm.K > mbg::kMaxReadouts) throw Error(MBG_ERR_LIMIT, "This executable supports a maximum of 20 non-internal nodes.");
         if (!(cm.time_tick > 0.f) || !(cm.max_time > 0.f)) throw Error(MBG_ERR_ARG, "max_time and time_tick must be positive");
-        auto V = std::make_unique<VariantRun>();
+        std::unique_ptr<VariantRun> V;
+        if (!c->variant_pool.empty()) {
+            V = std::move(c->variant_pool.back());
+            c->variant_pool.pop_back();
+            V->fixed_states.clear();
+            V->fixed_counts.clear();
+        } else {
+            V = std::make_unique<VariantRun>();
+        }
         V->m = m;
         V->n = vs[v].n_trajectories ? vs[v].n_trajectories : (uint64_t)cm.sample_count;
         if (V->n == 0 || V->n > (1ull << 32)) throw Error(MBG_ERR_ARG, "invalid trajectory count");
@@ -889,8 +901,9 @@ mbg_status parse_impl(const std::string& bnd, const std::string& cfg, mbg_model*
     *out = nullptr;
     auto m = std::make_unique<mbg_model>();
     try {
-        mbg::parse_model(bnd, cfg, m->parsed);
-        m->cm = mbg::compile_model(m->parsed);
+        m->parsed = std::make_shared<mbg::ParsedModel>();
+        mbg::parse_model(bnd, cfg, *m->parsed);
+        m->cm = mbg::compile_model(*m->parsed);
     } catch (const mbg::ParseError& e) {
         copy_err(err, errlen, e.what());
         return MBG_ERR_PARSE;
@@ -931,6 +944,23 @@ mbg_status mbg_model_parse_files(const char* bnd_path, const char* cfg_path, mbg
 }
 
 void mbg_model_free(mbg_model* m) { delete m; }
+
+mbg_status mbg_model_clone(const mbg_model* m, mbg_model** out) {
+    if (!m || !out) return MBG_ERR_ARG;
+    try {
+        auto c = std::make_unique<mbg_model>();
+        c->parsed = m->parsed;
+        c->cm = m->cm;
+        c->mult_up = m->mult_up;
+        c->mult_down = m->mult_down;
+        c->istate = m->istate;
+        *out = c.release();
+    } catch (const std::bad_alloc&) {
+        *out = nullptr;
+        return MBG_ERR_HIP;
+    }
+    return MBG_OK;
+}
 int mbg_model_num_nodes(const mbg_model* m) { return m ? m->cm.N : 0; }
 const char* mbg_model_node_name(const mbg_model* m, int i) {
     return (m && i >= 0 && i < m->cm.N) ? m->cm.names[i].c_str() : nullptr;
@@ -948,8 +978,8 @@ int mbg_model_num_readouts(const mbg_model* m) { return m ? m->cm.K : 0; }
 
 mbg_status mbg_model_constant(const mbg_model* m, const char* name, double* out) {
     if (!m || !name || !out) return MBG_ERR_ARG;
-    auto it = m->parsed.constants.find(name);
-    if (it == m->parsed.constants.end()) return MBG_ERR_ARG;
+    auto it = m->parsed->constants.find(name);
+    if (it == m->parsed->constants.end()) return MBG_ERR_ARG;
     *out = it->second;
     return MBG_OK;
 }
