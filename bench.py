@@ -1,6 +1,6 @@
 """Throughput benchmark of the trajectory kernel + statistics stage on one GPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep] [--no-surrogate]
 
 Workload ("large"): a synthetic random Boolean network of 1,000 nodes (in-degree 2-4, random
 AND/OR/NOT formulas, rate_up/rate_down log-uniform in [0.1, 10] FP32, istate U[0, 1], 16
@@ -15,7 +15,19 @@ transition, W(N, d, k) = 72 + 2*ceil(N/64) + 2*min(N, 64) + (1 + d)(2k + 6) + 10
 (d = mean out-degree, k = mean in-degree), against 256 CUs x 64 lanes x 2.4 GHz = 39.3 T
 lane-ops/s, divided by the kernel's own HIP-event duration.  cpu_baseline: the CPU oracle
 (the reference algorithm: full O(N) rate recompute and linear flip scan per step) on this
-host's cores over trajectory ids 0-4095 of the same workload.
+host's cores over trajectory ids 0-4095 of the same workload.  roofline.traffic: HBM bytes per
+launch of the kernel from the FETCH_SIZE and WRITE_SIZE counter summary committed in
+profiles/mbg_simulate_large_traffic.txt (each counter scaled by what it reads for a copy kernel
+of known size), null when that file is absent.
+
+Two more objects ride on the same line:
+  sweep      256 variant networks of 50-2,000 nodes (rates scaled by log-normal(0, 0.5), 0-3
+             knockouts, 1k-64k trajectories each) in one HIP-graph launch: transitions/s and
+             variants/s of the second (warm) sweep.
+  surrogate  the sweep surrogate on the 200-node network (50,000 simulated training variants,
+             4-member BF16 MLP ensemble, 10,000 held-out variants, spread gate 0.01, judged
+             against 262,144-trajectory simulations): accepted fraction, errors, points/s of
+             screening against plain batched simulation.
 """
 import argparse
 import json
@@ -32,6 +44,93 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # non-packed 32-bit lane-ops, MI355X
 
 def algorithmic_ops(n, mean_in, mean_out):
     return 72 + 2 * math.ceil(n / 64) + 2 * min(n, 64) + (1 + mean_out) * (2 * mean_in + 6) + 10
+
+
+def traffic_from_profile(path=os.path.join(ROOT, "profiles", "mbg_simulate_large_traffic.txt")):
+    """HBM bytes per mbg_simulate launch from the counter summary: lines `key: value` with keys
+    fetch_kib, write_kib (per dispatch), fetch_factor, write_factor (copy-kernel calibration)."""
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            parts = line.strip().split(":")
+            if len(parts) == 2 and parts[0].strip() in ("fetch_kib", "write_kib", "fetch_factor", "write_factor"):
+                vals[parts[0].strip()] = float(parts[1])
+    if len(vals) != 4:
+        return None
+    return (vals["fetch_kib"] * vals["fetch_factor"] + vals["write_kib"] * vals["write_factor"]) * 1024.0
+
+
+def copy_calibration(nbytes=1 << 30):
+    """One device-to-device copy of nbytes (the counters' calibration kernel)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def sweep_bench(ctx, count=256):
+    from upmabossg_amd import Model, synth
+
+    nets = synth.sweep_variants(count)
+    models = [Model(n.bnd(), n.cfg()) for n in nets]
+    vs = [(m, n.sample_count, 1, v) for v, (m, n) in enumerate(zip(models, nets))]
+    ctx.run_variants(vs)  # JIT of the word classes, buffers
+    t0 = time.perf_counter()
+    info = ctx.run_variants(vs)
+    wall = time.perf_counter() - t0
+    return {
+        "workload": f"sweep: {count} variant networks of {min(n.n for n in nets)}-{max(n.n for n in nets)} nodes, "
+                    "rates x log-normal(0, 0.5), 0-3 knockouts, 1k-64k trajectories each, one HIP graph",
+        "variants": count,
+        "word_classes": info.classes,
+        "trajectories": int(info.trajectories),
+        "transitions": int(info.transitions),
+        "graph_ms": info.graph_ms,
+        "setup_ms": info.setup_ms,
+        "wall_ms": wall * 1e3,
+        "transitions_per_s": info.transitions / wall,
+        "transitions_per_s_graph": info.transitions / (info.graph_ms / 1e3),
+        "variants_per_s": count / wall,
+    }
+
+
+def surrogate_bench():
+    from upmabossg_amd import surrogate
+
+    rep = surrogate.run(log=lambda m: print(m, file=sys.stderr, flush=True))
+    sc = rep["screen"]
+    return {
+        "workload": rep["workload"],
+        "train_points": rep["train"]["points"],
+        "train_trajectories_per_point": rep["train"]["trajectories_per_point"],
+        "members": rep["fit"]["members"],
+        "fit_s": rep["fit"]["seconds"],
+        "test_points": sc["test_points"],
+        "spread_gate": sc["threshold_spread"],
+        "accepted_fraction": sc["accepted_fraction"],
+        "reference_trajectories": sc["reference_trajectories"],
+        "surrogate_max_abs_error": sc["surrogate_error"]["max"],
+        "surrogate_mean_max_abs_error": sc["surrogate_error"]["mean"],
+        "surrogate_rms_error": sc["surrogate_rms"],
+        "plain_trajectories_per_point": sc["plain_trajectories_per_point"],
+        "plain_max_abs_error": sc["plain"][str(sc["plain_trajectories_per_point"])]["max"],
+        "plain_points_per_s": sc["plain_points_per_s"],
+        "screened_points_per_s": sc["screened_points_per_s"],
+        "speedup_at_0.01": sc["speedup_at_gate"],
+        "matched_rms_trajectories": sc["matched_rms_trajectories"],
+        "surrogate_points_per_s": sc["surrogate_points_per_s"],
+        "speedup_at_matched_rms": sc["speedup_at_matched_rms"],
+        "roofline": {"bound": "mfma", "achieved": sc["inference_tflops"], "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": sc["inference_tflops"] / 2500.0},
+    }
 
 
 def cpu_baseline(net, count=4096):
@@ -52,6 +151,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nodes", type=int, default=1000)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device to device (reference kernel for the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -66,6 +169,8 @@ def main():
     mean_in = sum(len(i) for i in ins) / net.n
     ops = algorithmic_ops(net.n, mean_in, mean_in)
 
+    if args.calibrate_copy:
+        copy_calibration()
     with Context(0) as ctx:
         ctx.compile(model)
         for _ in range(args.warmup):
@@ -79,6 +184,14 @@ def main():
             kernel_ms.append(i.kernel_ms)
         wall = time.perf_counter() - t0
         info = ctx.info()
+        print(f"large: {transitions / wall:.4g} transitions/s over {args.steps} steps", file=sys.stderr, flush=True)
+        sweep = None
+        if not args.no_sweep:
+            try:
+                sweep = sweep_bench(ctx)
+            except Exception as e:  # reported, never fatal to the headline line
+                sweep = {"error": f"{type(e).__name__}: {e}"}
+            print(f"sweep: {sweep}", file=sys.stderr, flush=True)
 
     kern_s = sum(kernel_ms) / len(kernel_ms) / 1e3
     per_launch = transitions / args.steps
@@ -113,9 +226,16 @@ def main():
             "unit": "T lane-op/s",
             "frac": achieved / VALU_PEAK_TOPS,
             "ops_per_transition": ops,
-            "traffic": None,
+            "traffic": traffic_from_profile(),
         },
     }
+    if sweep is not None:
+        out["sweep"] = sweep
+    if not args.no_surrogate:
+        try:
+            out["surrogate"] = surrogate_bench()
+        except Exception as e:
+            out["surrogate"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(net)
     print(json.dumps(out), flush=True)

@@ -46,8 +46,14 @@ mbg_status mbg_mlp_load_dataset(mbg_mlp* m, const float* x, const float* y, size
 /* epochs over the device training set in shuffled mini-batches of `batch` rows (a multiple of 64);
  * Adam with learning rate lr (beta1 0.9, beta2 0.999, eps 1e-8); mean loss of the last epoch. */
 mbg_status mbg_mlp_train(mbg_mlp* m, int epochs, int batch, float lr, uint64_t shuffle_seed, float* last_loss);
-/* Predictions y [n][n_out] for x [n][n_in] (host buffers); kernel time in ms. */
+/* Predictions y [n][n_out] for x [n][n_in] (host buffers); device time of the pass in ms. */
 mbg_status mbg_mlp_predict(mbg_mlp* m, const float* x, size_t n, float* y, double* ms);
+/* Ensemble prediction over `count` networks of equal input and output widths on one device:
+ * mean [n][n_out] = the members' average, spread [n] (may be NULL) = per row, the largest
+ * population standard deviation over members of any output.  compute_ms = device time from the
+ * inputs resident in device memory to the reduced results, before the copy back. */
+mbg_status mbg_mlp_ensemble_predict(mbg_mlp* const* members, int count, const float* x, size_t n, float* mean,
+                                    float* spread, double* compute_ms);
 
 #ifdef __cplusplus
 }

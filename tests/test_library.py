@@ -12,8 +12,8 @@ from tests.conftest import REFERENCE_MODELS, ROOT, model_text
 from upmabossg_amd import CLI_PATH, LIB_PATH, MbgError, Model, synth
 
 
-def _declared_symbols():
-    with open(os.path.join(ROOT, "include", "mbg.h")) as f:
+def _declared_symbols(header="mbg.h"):
+    with open(os.path.join(ROOT, "include", header)) as f:
         text = f.read()
     return sorted(set(re.findall(r"\b(mbg_[a-z_]+)\s*\(", text)))
 
@@ -25,6 +25,19 @@ def test_library_exports_every_declared_symbol():
     for s in syms:
         assert hasattr(lib, s), s
     out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (mbg_[a-z_]+)$", out, re.M))
+    assert exported == set(syms)
+
+
+def test_mlp_library_exports_every_declared_symbol():
+    from upmabossg_amd.mlp import MLP_LIB_PATH
+
+    syms = _declared_symbols("mbg_mlp.h")
+    assert "mbg_mlp_ensemble_predict" in syms and len(syms) >= 12
+    lib = ctypes.CDLL(MLP_LIB_PATH)
+    for s in syms:
+        assert hasattr(lib, s), s
+    out = subprocess.run(["nm", "-D", "--defined-only", MLP_LIB_PATH], capture_output=True, text=True).stdout
     exported = set(re.findall(r" T (mbg_[a-z_]+)$", out, re.M))
     assert exported == set(syms)
 

@@ -96,3 +96,19 @@ def test_gpu_training_reduces_loss():
     assert last < 0.25 * first
     pred, _ = m.predict(x[:512])
     assert np.mean((pred - y[:512]) ** 2) < 0.25 * first
+
+
+@pytest.mark.gpu
+def test_gpu_ensemble_predict_matches_members():
+    from upmabossg_amd.mlp import ensemble_predict
+
+    members = [_mlp(seed=s, n_in=100, hidden=128, layers=2, n_out=64, batch=128) for s in (7, 8, 9)]
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(300, 100)).astype(np.float32)  # 300 rows: three batches, the last one ragged
+    mean, spread, ms = ensemble_predict(members, x)
+    P = np.stack([m.predict(x)[0] for m in members])
+    assert ms > 0
+    assert np.max(np.abs(mean - P.mean(axis=0))) < 1e-6
+    assert np.max(np.abs(spread - P.std(axis=0).max(axis=1))) < 1e-6
+    one, s1, _ = ensemble_predict(members[:1], x)
+    assert np.array_equal(one, P[0]) and not s1.any()

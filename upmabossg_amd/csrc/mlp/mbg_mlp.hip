@@ -175,6 +175,45 @@ __global__ void to_bf16(const float* __restrict__ x, bf16* __restrict__ y, long 
     if (i < n) y[i] = (bf16)x[i];
 }
 
+// H[r][c] = BF16(X[r][c]) for r < rows, c < n_in; zero in the padding columns and in rows
+// rows..padded_rows-1.
+__global__ void pad_bf16(const float* __restrict__ X, bf16* __restrict__ H, long rows, long padded_rows, int n_in,
+                         int in_p) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= padded_rows * in_p) return;
+    const long r = i / in_p;
+    const int c = (int)(i - r * in_p);
+    H[i] = (r < rows && c < n_in) ? (bf16)X[r * n_in + c] : (bf16)0.f;
+}
+
+// One block per row: mean over `count` members' predictions P[k][row][c], and the row's largest
+// population standard deviation over members of any output.
+__global__ __launch_bounds__(256) void ensemble_reduce(const float* __restrict__ P, int count, long n, int n_out,
+                                                       float* __restrict__ mean, float* __restrict__ spread) {
+    const long row = blockIdx.x;
+    __shared__ float red[256];
+    float worst = 0.f;
+    for (int c = threadIdx.x; c < n_out; c += 256) {
+        float s = 0.f;
+        for (int k = 0; k < count; k++) s += P[((long)k * n + row) * n_out + c];
+        const float mu = s / (float)count;
+        float v = 0.f;
+        for (int k = 0; k < count; k++) {
+            const float d = P[((long)k * n + row) * n_out + c] - mu;
+            v += d * d;
+        }
+        mean[row * n_out + c] = mu;
+        worst = fmaxf(worst, sqrtf(v / (float)count));
+    }
+    red[threadIdx.x] = worst;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) spread[row] = red[0];
+}
+
 struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
@@ -221,6 +260,7 @@ struct mbg_mlp {
     Buf<bf16> Xset;                        // training set
     Buf<float> Yset;
     Buf<uint32_t> idx;
+    Buf<float> Xin, Pens, Mean, Spread;    // ensemble prediction: inputs, member outputs, results
     size_t nset = 0;
     long step = 0;
 
@@ -439,13 +479,14 @@ mbg_status mbg_mlp_gradients(mbg_mlp* m, const float* x, const float* y, int bat
 mbg_status mbg_mlp_load_dataset(mbg_mlp* m, const float* x, const float* y, size_t n) {
     if (!m || !x || !y || n == 0) return 3;
     return guarded(m, [&] {
-        std::vector<bf16> xb(n * m->in_p, (bf16)0.f);
-        for (size_t r = 0; r < n; r++)
-            for (int c = 0; c < m->n_in; c++) xb[r * m->in_p + c] = (bf16)x[r * m->n_in + c];
-        m->Xset.alloc(xb.size());
+        m->Xin.alloc(n * m->n_in);
+        m->Xset.alloc(n * m->in_p);
         m->Yset.alloc(n * m->n_out);
-        check(hipMemcpy(m->Xset.p, xb.data(), xb.size() * 2, hipMemcpyHostToDevice), "hipMemcpy");
+        check(hipMemcpy(m->Xin.p, x, n * m->n_in * 4, hipMemcpyHostToDevice), "hipMemcpy");
         check(hipMemcpy(m->Yset.p, y, n * m->n_out * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        pad_bf16<<<blocks((long)n * m->in_p), 256, 0, m->s>>>(m->Xin.p, m->Xset.p, (long)n, (long)n, m->n_in, m->in_p);
+        check(hipGetLastError(), "pad_bf16 launch");
+        check(hipStreamSynchronize(m->s), "load_dataset");
         m->nset = n;
         m->idx.alloc(n);
     });
@@ -479,33 +520,63 @@ mbg_status mbg_mlp_train(mbg_mlp* m, int epochs, int batch, float lr, uint64_t s
     });
 }
 
+mbg_status mbg_mlp_ensemble_predict(mbg_mlp* const* members, int count, const float* x, size_t n, float* mean,
+                                    float* spread, double* compute_ms) {
+    if (!members || count <= 0 || !members[0] || !x || !mean || n == 0) return 3;
+    mbg_mlp* m0 = members[0];
+    for (int k = 0; k < count; k++) {
+        const mbg_mlp* mk = members[k];
+        if (!mk || mk->device != m0->device || mk->n_in != m0->n_in || mk->n_out != m0->n_out) {
+            m0->err = "ensemble members must share the device and the input and output widths";
+            return 3;
+        }
+    }
+    return guarded(m0, [&] {
+        const int n_in = m0->n_in, n_out = m0->n_out;
+        m0->Xin.alloc(n * n_in);
+        m0->Pens.alloc((size_t)count * n * n_out);
+        m0->Mean.alloc(n * n_out);
+        m0->Spread.alloc(n);
+        check(hipMemcpyAsync(m0->Xin.p, x, n * n_in * 4, hipMemcpyHostToDevice, m0->s), "hipMemcpyAsync");
+        std::vector<hipEvent_t> ev(count + 2);
+        for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        check(hipEventRecord(ev[0], m0->s), "hipEventRecord");
+        for (int k = 0; k < count; k++) {
+            mbg_mlp* mk = members[k];
+            check(hipStreamWaitEvent(mk->s, ev[0], 0), "hipStreamWaitEvent");
+            for (size_t r0 = 0; r0 < n; r0 += mk->max_batch) {
+                const long rows = (long)std::min<size_t>(mk->max_batch, n - r0);
+                const long padded = (rows + 63) / 64 * 64;
+                pad_bf16<<<blocks(padded * mk->in_p), 256, 0, mk->s>>>(m0->Xin.p + r0 * n_in, mk->H[0].p, rows, padded,
+                                                                      n_in, mk->in_p);
+                mk->forward(padded, false);
+                check(hipMemcpyAsync(m0->Pens.p + ((size_t)k * n + r0) * n_out, mk->Yhat.p, (size_t)rows * n_out * 4,
+                                     hipMemcpyDeviceToDevice, mk->s),
+                      "hipMemcpyAsync");
+            }
+            check(hipEventRecord(ev[k + 1], mk->s), "hipEventRecord");
+            check(hipStreamWaitEvent(m0->s, ev[k + 1], 0), "hipStreamWaitEvent");
+        }
+        ensemble_reduce<<<(unsigned)n, 256, 0, m0->s>>>(m0->Pens.p, count, (long)n, n_out, m0->Mean.p, m0->Spread.p);
+        check(hipGetLastError(), "ensemble_reduce launch");
+        check(hipEventRecord(ev[count + 1], m0->s), "hipEventRecord");
+        check(hipMemcpyAsync(mean, m0->Mean.p, n * n_out * 4, hipMemcpyDeviceToHost, m0->s), "hipMemcpyAsync");
+        if (spread) check(hipMemcpyAsync(spread, m0->Spread.p, n * 4, hipMemcpyDeviceToHost, m0->s), "hipMemcpyAsync");
+        check(hipStreamSynchronize(m0->s), "ensemble predict");
+        float t = 0.f;
+        check(hipEventElapsedTime(&t, ev[0], ev[count + 1]), "hipEventElapsedTime");
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        if (compute_ms) *compute_ms = t;
+    });
+}
+
 mbg_status mbg_mlp_predict(mbg_mlp* m, const float* x, size_t n, float* y, double* ms) {
     if (!m || !x || !y) return 3;
-    return guarded(m, [&] {
-        hipEvent_t e0, e1;
-        check(hipEventCreate(&e0), "hipEventCreate");
-        check(hipEventCreate(&e1), "hipEventCreate");
-        float total = 0.f;
-        for (size_t r0 = 0; r0 < n; r0 += m->max_batch) {
-            const long rows = (long)std::min<size_t>(m->max_batch, n - r0);
-            const long padded = (rows + 63) / 64 * 64;
-            std::vector<float> xr((size_t)padded * m->n_in, 0.f);
-            std::memcpy(xr.data(), x + r0 * m->n_in, (size_t)rows * m->n_in * 4);
-            m->upload_batch(xr.data(), nullptr, padded);
-            check(hipEventRecord(e0, m->s), "hipEventRecord");
-            m->forward(padded, false);
-            check(hipEventRecord(e1, m->s), "hipEventRecord");
-            check(hipMemcpyAsync(y + r0 * m->n_out, m->Yhat.p, (size_t)rows * m->n_out * 4, hipMemcpyDeviceToHost, m->s),
-                  "hipMemcpyAsync");
-            check(hipStreamSynchronize(m->s), "predict");
-            float t = 0.f;
-            check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
-            total += t;
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (ms) *ms = total;
-    });
+    if (n == 0) {
+        if (ms) *ms = 0.0;
+        return 0;
+    }
+    return mbg_mlp_ensemble_predict(&m, 1, x, n, y, nullptr, ms);
 }
 
 }  // extern "C"

@@ -39,6 +39,8 @@ def lib():
         "mbg_mlp_load_dataset": (C.c_int, [vp, f32p, f32p, C.c_size_t]),
         "mbg_mlp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, f32p]),
         "mbg_mlp_predict": (C.c_int, [vp, f32p, C.c_size_t, f32p, C.POINTER(C.c_double)]),
+        "mbg_mlp_ensemble_predict": (C.c_int, [C.POINTER(vp), C.c_int, f32p, C.c_size_t, f32p, f32p,
+                                               C.POINTER(C.c_double)]),
     }
     for name, (rt, args) in sig.items():
         f = getattr(L, name)
@@ -133,3 +135,18 @@ class Mlp:
         ms = C.c_double()
         self._check(lib().mbg_mlp_predict(self._h, _fp(x), x.shape[0], _fp(y), C.byref(ms)))
         return y, ms.value
+
+
+def ensemble_predict(members, x):
+    """(mean [n][n_out], spread [n], device ms) of networks `members` on rows x [n][n_in]."""
+    x = np.ascontiguousarray(x, np.float32)
+    n = x.shape[0]
+    out = members[0].n_out
+    mean = np.zeros((n, out), np.float32)
+    spread = np.zeros(n, np.float32)
+    ms = C.c_double()
+    handles = (C.c_void_p * len(members))(*[m._h.value for m in members])
+    st = lib().mbg_mlp_ensemble_predict(handles, len(members), _fp(x), n, _fp(mean), _fp(spread), C.byref(ms))
+    if st != 0:
+        raise MlpError(lib().mbg_mlp_last_error(members[0]._h).decode())
+    return mean, spread, ms.value
