@@ -1,0 +1,144 @@
+"""The MaBoSSG CLI end to end on the GPU: its <prefix>_probtraj.csv, <prefix>_fp.csv and stdout
+text are compared byte for byte with text formatted here, independently, from the oracle's
+tables of the same run (the GPU's tables equal the oracle's bit for bit, so any difference is
+a formatting difference).
+
+The formatter below restates the reference's writers:
+  probtraj  src/statistics/window_average_small.cpp:181-250 (header :192-215, rows :219-248)
+  stdout    src/statistics/final_states.cpp:53-66, src/statistics/fixed_states.cu:141-152,
+            src/statistics/window_average_small.cpp:142-179
+  fp        src/statistics/fixed_states.cu:154-187
+  state     src/state.cpp:29-48 (" -- " between ON node names, "<nil>" when none)
+with values printed as FP32 through the default ostream format (%g, 6 significant digits).
+Normalisation is this build's: cells are exact fixed-point sums, P = cell 2^-fp / (n tick)
+(counts / n in discrete time), TH = th 2^-ft / n (/ tick), evaluated in FP64 and printed as FP32.
+"""
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleModel
+from tests.conftest import model_text
+from upmabossg_amd import CLI_PATH
+
+pytestmark = pytest.mark.gpu
+
+_LIBM = ctypes.CDLL("libm.so.6")
+_LIBM.log2f.restype = ctypes.c_float
+_LIBM.log2f.argtypes = [ctypes.c_float]
+f32 = np.float32
+
+
+def g(v):
+    return "%g" % float(f32(v))
+
+
+class Expected:
+    def __init__(self, bnd, cfg):
+        o = OracleModel(bnd, cfg)
+        self.names = o.names()
+        internal = o.internal()
+        self.readouts = [i for i in range(len(self.names)) if not internal[i]]
+        self.discrete = o.constant("discrete_time") != 0
+        self.tick = f32(o.constant("time_tick"))
+        self.n = int(o.constant("sample_count"))
+        self.r = o.run(count=self.n, n_trajectories=self.n, threads=16, record_fixed=True)
+
+    def proj(self, s):
+        on = [self.names[self.readouts[j]] for j in range(len(self.readouts)) if (s >> j) & 1]
+        return " -- ".join(on) if on else "<nil>"
+
+    def full(self, words):
+        on = [self.names[i] for i in range(len(self.names)) if (int(words[i >> 6]) >> (i & 63)) & 1]
+        return " -- ".join(on) if on else "<nil>"
+
+    def prob(self, w, s):
+        v = float(self.r.prob[w, s])
+        if self.discrete:
+            return f32(v / self.n)
+        return f32(math.ldexp(v, -self.r.prob_exponent) / (self.n * float(self.tick)))
+
+    def th(self, w):
+        v = math.ldexp(float(self.r.th[w]), -self.r.th_exponent) / self.n
+        if not self.discrete:
+            v /= float(self.tick)
+        return f32(v)
+
+    def row(self, w):
+        probs, ent = [], f32(0)
+        for s in range(self.r.prob.shape[1]):
+            p = self.prob(w, s)
+            if p == 0:
+                continue
+            ent = f32(ent + f32(f32(-_LIBM.log2f(p)) * p))
+            probs.append((s, p))
+        return probs, ent
+
+    def probtraj_csv(self):
+        rows = [self.row(w) for w in range(self.r.windows)]
+        out = "Time\tTH\tErrorTH\tH\tHD=0" + "\tState\tProba\tErrorProba" * max(len(p) for p, _ in rows) + "\n"
+        for w, (probs, ent) in enumerate(rows):
+            out += "\t".join([g(f32(w) * self.tick), g(self.th(w)), "0", g(ent), "0"])
+            for s, p in probs:
+                out += f"\t{self.proj(s)}\t{g(p)}\t0"
+            out += "\n"
+        return out
+
+    def fp_csv(self):
+        F = len(self.r.fixed_counts)
+        out = f"Fixed Points ({F})\n" + "FP\tProba\tState" + "".join("\t" + n for n in self.names) + "\n"
+        for i in range(F):
+            st = self.r.fixed_states[i]
+            out += f"#{i}\t{g(f32(self.r.fixed_counts[i]) / f32(self.n))}\t{self.full(st)}"
+            out += "".join("\t%d" % ((int(st[k >> 6]) >> (k & 63)) & 1) for k in range(len(self.names))) + "\n"
+        return out
+
+    def text(self):
+        n = f32(self.n)
+        out = "final points:\n"
+        for s, c in enumerate(self.r.final):
+            if c:
+                out += f"{g(f32(c) / n)} {self.proj(s)}\n"
+        out += "fixed points:\n"
+        for i, c in enumerate(self.r.fixed_counts):
+            out += f"{g(f32(c) / n)} {self.full(self.r.fixed_states[i])}\n"
+        for w in range(self.r.windows):
+            probs, ent = self.row(w)
+            out += f"window ({g(f32(w) * self.tick)}, {g(f32(w + 1) * self.tick)}]\n"
+            out += f"entropy: {g(ent)}\ntransition entropy: {g(self.th(w))}\n"
+            for s, p in probs:
+                out += f"{g(p)} {self.proj(s)}\n"
+        return out
+
+
+CASES = {
+    "metastasis": lambda: model_text("metastasis"),  # continuous, 1,000 trajectories, fixed points
+    "cellcycle": lambda: (model_text("cellcycle")[0], model_text("cellcycle")[1] + "\nsample_count = 20000;\n"),
+    "frozen": lambda: ("node A { logic = A; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n"
+                       "node B { logic = !A; rate_up = @logic ? 2 : 0; rate_down = @logic ? 0 : 2; }\n",
+                       "A.istate = 0.5; B.istate = 0.5; max_time = 5; time_tick = 1; sample_count = 1000;"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_cli_output_bytes(tmp_path, name):
+    bnd, cfg = CASES[name]()
+    bp, cp = tmp_path / "m.bnd", tmp_path / "m.cfg"
+    bp.write_text(bnd)
+    cp.write_text(cfg)
+    print(f"case: CLI {name}", flush=True)
+    run = subprocess.run([CLI_PATH, "-o", str(tmp_path / "out"), str(bp), str(cp)], capture_output=True, text=True,
+                         timeout=120)
+    assert run.returncode == 0, run.stderr
+    text = subprocess.run([CLI_PATH, str(bp), str(cp)], capture_output=True, text=True, timeout=120)
+    assert text.returncode == 0, text.stderr
+    want = Expected(bnd, cfg)
+    assert (tmp_path / "out_probtraj.csv").read_text() == want.probtraj_csv()
+    assert (tmp_path / "out_fp.csv").read_text() == want.fp_csv()
+    assert text.stdout == want.text()
+    if name == "metastasis":
+        assert len(want.r.fixed_counts) > 0

@@ -329,6 +329,15 @@ class Context:
                          i.time_tick, i.max_time, prob_raw.reshape(W, S), th_raw, prob.reshape(W, S), th, fin,
                          fs, fc, summ)
 
+    def probtraj_raw(self, out=None) -> np.ndarray:
+        """The last run's window table (W * 2^K exact cells), copied to host memory."""
+        i = self.info()
+        n = i.windows * (1 << i.readouts)
+        if out is None or out.size != n:
+            out = np.zeros(n, np.uint64)
+        self._check(lib().mbg_get_probtraj_raw(self._h, _u64p(out), None))
+        return out
+
     def marginals(self) -> np.ndarray:
         """(W, K) probability that each readout is ON, per window, of the last run."""
         i = self.info()

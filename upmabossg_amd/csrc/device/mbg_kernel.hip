@@ -9,10 +9,12 @@
 //   MBG_TABLES_LDS  1 to stage the model tables in LDS (they fit), 0 to read them from HBM
 //   MBG_MAX_IN      largest number of inputs of a node (gathers are unrolled up to 6)
 //   MBG_DEP_BATCH   dependents evaluated per round of independent table reads
+//   MBG_SCAN        candidate nodes per round of the in-word flip search
 //   MBG_WSUM_REG    1: the word rate sums live in VGPRs (more lanes fit in LDS), 0: in LDS
 //   MBG_RATE32      1: every fixed-point rate fits 32 bits, the rate table is u32
 //   MBG_MULTI       1: emit the variant-sweep kernel instead of the single-model one
 //   MBG_SUMW        64-bit words per rate-sum group (register sums only; divides MBG_NW)
+//   MBG_COMPACT     1: 8-byte node records and 16-bit dependents (small models, no generic rates)
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -44,6 +46,9 @@ typedef unsigned int u32;
 typedef unsigned short u16;
 typedef unsigned long long u64;
 
+// Pointers into memory that no kernel writes while it runs (the model tables).
+#define CONST_AS __attribute__((address_space(4)))
+
 struct mbg_node {
     u32 in_off;               // first input in the inputs table
     u32 dep_off;              // first dependent in the deps table
@@ -57,6 +62,40 @@ struct mbg_node {
 #define F_GEN 0x80u
 #define F_SELF 0x40u
 #define F_RO 0x3fu
+
+#ifndef MBG_COMPACT
+#define MBG_COMPACT 0
+#endif
+
+// A node record decoded into registers (either table layout).
+struct mbg_nd {
+    u32 in_off, dep_off, tab_off, dep_cnt, in_cnt, flags;
+};
+
+// MBG_COMPACT layout: one u64 per node, bits 0-13 in_off, 14-27 dep_off, 28-43 tab_off,
+// 44-51 dep_cnt, 52-55 in_cnt, 56 F_SELF, 57-61 readout index + 1 (no generic nodes); the
+// dependents table holds u16 node (12 bits) | key bit position (4 bits).
+__device__ __forceinline__ mbg_nd decode_compact(u64 v) {
+    mbg_nd n;
+    n.in_off = (u32)v & 0x3fffu;
+    n.dep_off = (u32)(v >> 14) & 0x3fffu;
+    n.tab_off = (u32)(v >> 28) & 0xffffu;
+    n.dep_cnt = (u32)(v >> 44) & 0xffu;
+    n.in_cnt = (u32)(v >> 52) & 0xfu;
+    n.flags = (((u32)(v >> 56) & 1u) << 6) | ((u32)(v >> 57) & 0x1fu);
+    return n;
+}
+
+__device__ __forceinline__ mbg_nd decode_wide(const mbg_node& r) {
+    mbg_nd n;
+    n.in_off = r.in_off;
+    n.dep_off = r.dep_off;
+    n.tab_off = r.tab_off;
+    n.dep_cnt = r.dep_cnt;
+    n.in_cnt = r.in_cnt;
+    n.flags = r.flags;
+    return n;
+}
 
 // Byte offsets of the model tables inside one blob (global memory, or LDS when staged).
 struct mbg_layout {
@@ -98,13 +137,22 @@ struct mbg_desc {
     int N, K, W, FP, FT, sigma;
 };
 
-#define TNODE(i) (((const mbg_node*)(tb + D.lay.nodes))[i])
+#if MBG_COMPACT
+#define TNODE(i) decode_compact(((const u64*)(tb + D.lay.nodes))[i])
+#else
+#define TNODE(i) decode_wide(((const mbg_node*)(tb + D.lay.nodes))[i])
+#endif
 #if MBG_RATE32
 #define TRATE(i) ((u64)((const u32*)(tb + D.lay.rates))[i])
 #else
 #define TRATE(i) (((const u64*)(tb + D.lay.rates))[i])
 #endif
+#if MBG_COMPACT
+#define TDEP(i) dep_entry(((const unsigned short*)(tb + D.lay.deps))[i])
+__device__ __forceinline__ u32 dep_entry(u32 v) { return (v & 0xfffu) | ((v >> 12) << 24); }
+#else
 #define TDEP(i) (((const u32*)(tb + D.lay.deps))[i])
+#endif
 #define TIN(i) ((u32)((const u16*)(tb + D.lay.inputs))[i])
 #define TLUT(i) (((const u32*)(tb + D.lay.lut))[i])
 #define TRO(i) (((const u32*)(tb + D.lay.readouts))[i])
@@ -161,7 +209,7 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_state(mbg_desc D) {
     D.stage[t * 3 * MBG_NW + w] = initial_word(D, (u32)(D.batch_first + t), w);
 }
 
-__device__ __forceinline__ u64 rate_from_key(const mbg_desc& D, const char* tb, const mbg_node& n, u32 i, u32 key) {
+__device__ __forceinline__ u64 rate_from_key(const mbg_desc& D, const char* tb, const mbg_nd& n, u32 i, u32 key) {
 #if MBG_GENERIC
     if (n.flags & 0x80) return D.rtab[n.tab_off + key];
 #endif
@@ -195,21 +243,85 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
     __syncthreads();
     const u64 t = t0 + lane;
     const char* tb = D.blob;
-    for (int w = (int)(tid >> 6); w < MBG_NW; w += 4) {
+    // Eight nodes per round: their keys are gathered first and their table reads issued
+    // together, so the wave waits for one round of loads instead of one per node.  Node
+    // indices are wave-uniform (scalar table reads); past the last node they are clamped to
+    // it and the result discarded.
+    for (int w = (int)__builtin_amdgcn_readfirstlane(tid >> 6); w < MBG_NW; w += 4) {
         u64 a = 0, sum = 0;
-        for (int b = 0; b < 64; b++) {
-            const u32 i = (u32)(w * 64 + b);
-            if ((int)i >= D.N) break;
-            const mbg_node n = TNODE(i);
-            u32 key = (u32)(xs[(i >> 6) * INIT_TRAJ + lane] >> (i & 63)) & 1u;
-            for (u32 m = 0; m < n.in_cnt; m++) {
-                const u32 j = TIN(n.in_off + m);
-                key |= ((u32)(xs[(j >> 6) * INIT_TRAJ + lane] >> (j & 63)) & 1u) << (m + 1);
+        const int nb = min(64, D.N - w * 64);
+        for (int b0 = 0; b0 < nb; b0 += 8) {
+            u32 key[8], tab[8];
+            u64 rup[8], rdn[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const u32 i = (u32)min(w * 64 + b0 + q, D.N - 1);
+                // node record and input list through the constant address space: the
+                // index is wave-uniform, so these become scalar loads
+#if MBG_COMPACT
+                const mbg_nd nd = decode_compact(((const CONST_AS u64*)(tb + D.lay.nodes))[i]);
+                const u32 in_off = nd.in_off, in_cnt = nd.in_cnt;
+                tab[q] = nd.tab_off;
+#else
+                const CONST_AS u32* nr = (const CONST_AS u32*)(tb + D.lay.nodes) + 4 * i;
+                const u32 in_off = nr[0], in_cnt = nr[3] >> 16 & 0xffu;
+                tab[q] = nr[2];
+#endif
+#if !MBG_GENERIC
+#if MBG_RATE32
+                const CONST_AS u32* rt = (const CONST_AS u32*)(tb + D.lay.rates);
+#else
+                const CONST_AS u64* rt = (const CONST_AS u64*)(tb + D.lay.rates);
+#endif
+                rup[q] = rt[2 * i];
+                rdn[q] = rt[2 * i + 1];
+#endif
+                u32 k = (u32)(xs[(i >> 6) * INIT_TRAJ + lane] >> (i & 63)) & 1u;
+                const CONST_AS u32* ins = (const CONST_AS u32*)(tb + D.lay.inputs);
+#if MBG_MAX_IN <= 6
+                // the node's (up to 6) 16-bit input ids in one 16-byte scalar load; reads
+                // past its inputs land in the next node's entries or the table's padding
+                // and are masked off
+                const u32 b4 = in_off >> 1, odd = in_off & 1u;
+                const u32 iw[4] = {ins[b4], ins[b4 + 1], ins[b4 + 2], ins[b4 + 3]};
+#pragma unroll
+                for (u32 m = 0; m < MBG_MAX_IN; m++) {
+                    const u32 loc = odd + m;
+                    const u32 j = (iw[loc >> 1] >> ((loc & 1u) * 16u)) & 0xffffu;
+                    const u32 b = (u32)(xs[(j >> 6) * INIT_TRAJ + lane] >> (j & 63)) & 1u;
+                    k |= (m < in_cnt ? b : 0u) << (m + 1);
+                }
+#else
+                for (u32 m = 0; m < in_cnt; m++) {
+                    const u32 e = in_off + m;
+                    const u32 j = (ins[e >> 1] >> ((e & 1u) * 16u)) & 0xffffu;
+                    k |= ((u32)(xs[(j >> 6) * INIT_TRAJ + lane] >> (j & 63)) & 1u) << (m + 1);
+                }
+#endif
+                key[q] = k;
             }
-            const u64 r = rate_from_key(D, tb, n, i, key);
-            if (r) {
-                a |= 1ull << b;
-                sum += r;
+            u64 r[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const u32 i = (u32)min(w * 64 + b0 + q, D.N - 1);
+#if MBG_GENERIC
+                u64 v = rate_from_key(D, tb, TNODE(i), i, key[q]);
+#else
+                // fast form: the truth-table bit per lane, the two rates of the (uniform) node
+                const u32 bp = tab[q] + key[q];
+                const u32 logic = (TLUT(bp >> 5) >> (bp & 31)) & 1u;
+                const u32 xi = key[q] & 1u;
+                const u64 v = (xi ? rdn[q] : rup[q]) & (0ull - (u64)(xi != logic));
+                (void)i;
+#endif
+                r[q] = b0 + q < nb ? v : 0ull;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (r[q]) {
+                    a |= 1ull << (b0 + q);
+                    sum += r[q];
+                }
             }
         }
         if (t < D.batch) {
@@ -230,7 +342,7 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
     for (int b = 0; b < 64; b++) {
         const u32 i = (u32)(w * 64 + b);
         if ((int)i >= D.N) break;
-        const mbg_node n = TNODE(i);
+        const mbg_nd n = TNODE(i);
         u32 key = (u32)(x[i >> 6] >> (i & 63)) & 1u;
         for (u32 m = 0; m < n.in_cnt; m++) {
             const u32 j = TIN(n.in_off + m);
@@ -257,7 +369,7 @@ struct Lane {
 
     __device__ __forceinline__ u32 xbit(u32 j) const { return (u32)(LANE_X(j >> 6) >> (j & 63)) & 1u; }
 
-    __device__ __forceinline__ u32 key_of(const mbg_node& n, u32 i) const {
+    __device__ __forceinline__ u32 key_of(const mbg_nd& n, u32 i) const {
         u32 key = xbit(i);
 #if MBG_MAX_IN <= 6
         // branch-free: reads past a node's inputs hit the next node's entries (or the
@@ -275,7 +387,7 @@ struct Lane {
 
     // Entropy term rint(r log2(r) 2^sigma) of readout ro (index) at rate r produced with
     // own state bit xi: a table entry for fast nodes, computed for generic ones.
-    __device__ __forceinline__ u64 term(const mbg_node& n, u32 ro, u32 xi, u64 r) const {
+    __device__ __forceinline__ u64 term(const mbg_nd& n, u32 ro, u32 xi, u64 r) const {
         if (!r) return 0ull;
 #if MBG_GENERIC
         if (n.flags & F_GEN) {
@@ -356,7 +468,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                         const u32 n = TRO(j);
                         const u32 xi = L.xbit(n);
                         s |= xi << j;
-                        const mbg_node nd = TNODE(n);
+                        const mbg_nd nd = TNODE(n);
                         u64 r;
 #if MBG_GENERIC
                         if (nd.flags & F_GEN) r = rate_from_key(D, tb, nd, n, L.key_of(nd, n));
@@ -528,24 +640,24 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #endif
             const u32 wbase = wi * 64u;
             while (m) {
-                // four candidates per round trip to the rate table
-                u32 bb[4];
-                u64 rr[4], mm = m;
+                // MBG_SCAN candidates per round trip to the rate table
+                u32 bb[MBG_SCAN];
+                u64 rr[MBG_SCAN], mm = m;
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
+                for (int c = 0; c < MBG_SCAN; c++) {
                     bb[c] = mm ? (u32)__builtin_ctzll(mm) : 64u;
                     mm = mm ? (mm & (mm - 1ull)) : 0ull;
                     const u32 b = bb[c] < 64u ? bb[c] : 0u;
                     rr[c] = bb[c] < 64u ? TRATE(2 * (wbase + b) + ((xq >> b) & 1ull)) : 0ull;
 #if MBG_GENERIC
                     if (bb[c] < 64u && ((gm >> b) & 1ull)) {
-                        const mbg_node nd = TNODE(wbase + b);
+                        const mbg_nd nd = TNODE(wbase + b);
                         rr[c] = rate_from_key(D, tb, nd, wbase + b, L.key_of(nd, wbase + b));
                     }
 #endif
                 }
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
+                for (int c = 0; c < MBG_SCAN; c++) {
                     if (!hit && bb[c] < 64u) {
                         if (rem < rr[c]) {
                             j = wbase + bb[c];
@@ -565,7 +677,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         // ---- flip j and update the exact sums ----
         const u32 xj_old = (u32)(xw >> (j & 63)) & 1u;
         LANE_X(j >> 6) = xw ^ (1ull << (j & 63));
-        const mbg_node nj = TNODE(j);
+        const mbg_nd nj = TNODE(j);
         if (!(nj.flags & F_SELF)) {
             // fast node that does not read itself: it was active, it now agrees with its
             // logic, so its rate drops from r_sel to 0
@@ -584,7 +696,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         for (u32 q0 = 0; q0 < nj.dep_cnt; q0 += DEP_BATCH) {
             u32 kk[DEP_BATCH], xo[DEP_BATCH], xn[DEP_BATCH];
             u64 rn[DEP_BATCH], rp[DEP_BATCH];
-            mbg_node nk[DEP_BATCH];
+            mbg_nd nk[DEP_BATCH];
 #pragma unroll
             for (int u = 0; u < DEP_BATCH; u++) {
                 const bool ok = q0 + u < nj.dep_cnt;

@@ -96,11 +96,11 @@ using mbg_desc_ptr = const void*;
 
 struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 4, scan = 4, wsum_reg = 0, rate32 = 0,
-        multi = 0, sumw = 1;
+        multi = 0, sumw = 1, compact = 0;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
-                        sumw) < std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in,
-                                         o.dep_batch, o.scan, o.wsum_reg, o.rate32, o.multi, o.sumw);
+                        sumw, compact) < std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in,
+                                                  o.dep_batch, o.scan, o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact);
     }
 };
 
@@ -116,10 +116,12 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_TABLES_LDS " << sh.tables_lds << "\n";
     os << "#define MBG_MAX_IN " << sh.max_in << "\n";
     os << "#define MBG_DEP_BATCH " << sh.dep_batch << "\n";
+    os << "#define MBG_SCAN " << sh.scan << "\n";
     os << "#define MBG_WSUM_REG " << sh.wsum_reg << "\n";
     os << "#define MBG_RATE32 " << sh.rate32 << "\n";
     os << "#define MBG_MULTI " << sh.multi << "\n";
     os << "#define MBG_SUMW " << sh.sumw << "\n";
+    os << "#define MBG_COMPACT " << sh.compact << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -137,7 +139,17 @@ bool rates_fit_u32(const mbg::RateTables& rt) {
     return true;
 }
 
-Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt, bool rate32) {
+// The compact table layout (8-byte node records, 16-bit dependents) fits when the model has
+// no generic rate tables and its tables stay within the packed field widths.
+bool compact_fits(const mbg::CompiledModel& cm) {
+    if (cm.any_generic || cm.N > 4096 || cm.K > 31 || cm.max_inputs > 15) return false;
+    if (cm.inputs.size() + 32 >= (1u << 14) || cm.deps.size() >= (1u << 14)) return false;
+    for (const mbg::NodeRec& r : cm.nodes)
+        if (r.dep_cnt > 255 || (uint64_t)r.tab_off + (2ull << r.in_cnt) > (1ull << 16)) return false;
+    return true;
+}
+
+Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt, bool rate32, bool compact) {
     Blob b;
     std::vector<uint8_t> bytes;
     auto seg = [&](const void* p, size_t n) -> uint32_t {
@@ -149,14 +161,35 @@ Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt, bool rat
     std::vector<uint16_t> in16(cm.inputs.begin(), cm.inputs.end());
     in16.resize(in16.size() + 32, 0);  // the kernel's unrolled gathers may read past the last node
     std::vector<uint32_t> ro(cm.readouts.begin(), cm.readouts.end());
-    b.lay.nodes = seg(cm.nodes.data(), cm.nodes.size() * sizeof(mbg::NodeRec));
+    if (compact) {
+        // bits 0-13 in_off, 14-27 dep_off, 28-43 tab_off, 44-51 dep_cnt, 52-55 in_cnt,
+        // 56 listed among its own dependents, 57-61 readout index + 1
+        std::vector<uint64_t> nodes(cm.nodes.size());
+        for (size_t i = 0; i < nodes.size(); i++) {
+            const mbg::NodeRec& r = cm.nodes[i];
+            nodes[i] = (uint64_t)r.in_off | (uint64_t)r.dep_off << 14 | (uint64_t)r.tab_off << 28 |
+                       (uint64_t)r.dep_cnt << 44 | (uint64_t)r.in_cnt << 52 | (uint64_t)((r.flags >> 6) & 1u) << 56 |
+                       (uint64_t)(r.flags & 0x3fu) << 57;
+        }
+        b.lay.nodes = seg(nodes.data(), nodes.size() * 8);
+    } else {
+        b.lay.nodes = seg(cm.nodes.data(), cm.nodes.size() * sizeof(mbg::NodeRec));
+    }
     if (rate32) {
         std::vector<uint32_t> r32(rt.rates.begin(), rt.rates.end());
         b.lay.rates = seg(r32.data(), r32.size() * 4);
     } else {
         b.lay.rates = seg(rt.rates.data(), rt.rates.size() * 8);
     }
-    b.lay.deps = seg(cm.deps.data(), cm.deps.size() * 4);
+    if (compact) {
+        // node id (12 bits) | key bit position (4 bits)
+        std::vector<uint16_t> d16(cm.deps.size());
+        for (size_t i = 0; i < d16.size(); i++)
+            d16[i] = (uint16_t)((cm.deps[i] & 0xfffu) | ((cm.deps[i] >> 24) & 0xfu) << 12);
+        b.lay.deps = seg(d16.data(), d16.size() * 2);
+    } else {
+        b.lay.deps = seg(cm.deps.data(), cm.deps.size() * 4);
+    }
     b.lay.inputs = seg(in16.data(), in16.size() * 2);
     b.lay.lut = seg(cm.lut.data(), cm.lut.size() * 4);
     b.lay.readouts = seg(ro.data(), ro.size() * 4);
@@ -254,6 +287,7 @@ struct mbg_ctx {
     bool have = false;
     mbg_run_info info{};
     mbg::PinnedU64 prob;  // window table, pinned for a fast device-to-host copy
+    bool prob_on_host = false;  // the window table stays on the device until a getter asks
     std::vector<uint64_t> th, fin, fixed_states, fixed_counts, summ;
 };
 
@@ -340,6 +374,7 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
         }
     }
     best.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 4)));
+    best.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
     best.sumw = default_sumw(cm.NW, best.wsum_reg);
     if (best_score < 0) {
         if (block_req) throw Error(MBG_ERR_LIMIT, "the requested block does not fit in LDS for this model");
@@ -415,8 +450,10 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
                                                      m->mult_down.empty() ? nullptr : m->mult_down.data(),
                                                      m->istate.empty() ? nullptr : m->istate.data());
     const bool rate32 = rates_fit_u32(rt);
-    const Blob blob = pack_blob(cm, rt, rate32);
-    const Shape sh = choose_shape(c, cm, blob.lay.bytes, p->block, p->trace, rate32 ? 1 : 0);
+    const bool compact = compact_fits(cm) && env_int("MBG_TUNE_COMPACT", 1) != 0;
+    const Blob blob = pack_blob(cm, rt, rate32, compact);
+    Shape sh = choose_shape(c, cm, blob.lay.bytes, p->block, p->trace, rate32 ? 1 : 0);
+    sh.compact = compact ? 1 : 0;
     const Module& mod = get_module(c, cm, sh);
     const size_t smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0);
     int blocks_per_cu = 0;
@@ -527,10 +564,10 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 
-    c->prob.resize((size_t)W * S);
+    c->prob.resize((size_t)W * S);  // filled from the device by the first getter that needs it
+    c->prob_on_host = false;
     c->th.resize(W);
     c->fin.resize(S);
-    hip_check(hipMemcpy(c->prob.data(), d_prob.p, c->prob.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     hip_check(hipMemcpy(c->th.data(), d_th.p, c->th.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     hip_check(hipMemcpy(c->fin.data(), d_fin.p, c->fin.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     uint64_t nfp = 0, flips = 0;
@@ -631,6 +668,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             sh = Shape{nwc, 64, cm.discrete ? 1 : 0, 0, 0, 0, 1};
             sh.rate32 = 1;
             sh.multi = 1;
+            sh.compact = env_int("MBG_TUNE_COMPACT", 1) != 0 ? 1 : 0;
             sh.wsum_reg = nwc <= 16 ? 1 : 0;
             sh.sumw = default_sumw(nwc, sh.wsum_reg);
         }
@@ -639,6 +677,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         C.sh.generic |= cm.any_generic ? 1 : 0;
         C.sh.max_in = std::max(C.sh.max_in, cm.max_inputs);
         C.sh.rate32 &= fits32[v];
+        C.sh.compact &= compact_fits(cm) ? 1 : 0;
         sw->vars.push_back(std::move(V));
     }
     // per class: block size for the lanes' LDS, module, occupancy
@@ -657,6 +696,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         }
         if (!best_waves) throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
         C.sh.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 4)));
+        C.sh.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
         const Module& mod = get_module(c, sw->vars[C.members[0]]->m->cm, C.sh);
         C.sim = mod.sim;
         C.init_state = mod.init_state;
@@ -675,7 +715,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             VariantRun& V = *sw->vars[v];
             const mbg::CompiledModel& cm = V.m->cm;
             const mbg::RateTables& rt = rts[v];
-            const Blob blob = pack_blob(cm, rt, C.sh.rate32 != 0);
+            const Blob blob = pack_blob(cm, rt, C.sh.rate32 != 0, C.sh.compact != 0);
             const RunScales rs = run_scales(cm, V.n);
             const uint64_t fp_cap = V.n;
             V.marginals = C.marginals;
@@ -857,7 +897,20 @@ void normalise_marginals(const mbg_run_info& I, const std::vector<uint64_t>& raw
     }
 }
 
+// The window table of the last run on the host: copied from the device on first use (a run
+// leaves it in device memory, so callers that only need device-side results or the small
+// tables never pay for the transfer).
+void ensure_prob(const mbg_ctx* cc) {
+    mbg_ctx* c = const_cast<mbg_ctx*>(cc);
+    if (c->prob_on_host) return;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    if (!c->prob.empty())
+        hip_check(hipMemcpy(c->prob.data(), c->buf.prob.p, c->prob.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    c->prob_on_host = true;
+}
+
 mbg::Results results_view(const mbg_ctx* c) {
+    ensure_prob(c);
     mbg::Results r;
     r.info = &c->info;
     r.prob = &c->prob;
@@ -1055,7 +1108,10 @@ mbg_status mbg_compile(mbg_ctx* c, const mbg_model* m, int trace) {
         const mbg::CompiledModel& cm = m->cm;
         const mbg::RateTables rt = mbg::make_rate_tables(cm, nullptr, nullptr, nullptr);
         const bool r32 = rates_fit_u32(rt);
-        get_module(c, cm, choose_shape(c, cm, pack_blob(cm, rt, r32).lay.bytes, 0, trace, r32 ? 1 : 0));
+        const bool compact = compact_fits(cm) && env_int("MBG_TUNE_COMPACT", 1) != 0;
+        Shape sh = choose_shape(c, cm, pack_blob(cm, rt, r32, compact).lay.bytes, 0, trace, r32 ? 1 : 0);
+        sh.compact = compact ? 1 : 0;
+        get_module(c, cm, sh);
     });
 }
 
@@ -1078,20 +1134,25 @@ mbg_status mbg_get_run_info(const mbg_ctx* c, mbg_run_info* out) {
 mbg_status mbg_get_probtraj_raw(const mbg_ctx* c, uint64_t* prob, uint64_t* th) {
     if (!c) return MBG_ERR_ARG;
     if (!c->have) return MBG_ERR_NOT_BUILT;
-    if (prob) std::memcpy(prob, c->prob.data(), c->prob.size() * 8);
-    if (th) std::memcpy(th, c->th.data(), c->th.size() * 8);
-    return MBG_OK;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        if (prob) {
+            ensure_prob(c);
+            std::memcpy(prob, c->prob.data(), c->prob.size() * 8);
+        }
+        if (th) std::memcpy(th, c->th.data(), c->th.size() * 8);
+    });
 }
 
 mbg_status mbg_get_probtraj(const mbg_ctx* c, double* prob, double* th) {
     if (!c) return MBG_ERR_ARG;
     if (!c->have) return MBG_ERR_NOT_BUILT;
-    const mbg::Results r = results_view(c);
-    if (prob)
-        for (size_t i = 0; i < c->prob.size(); i++) prob[i] = mbg::window_prob(r, i);
-    if (th)
-        for (size_t w = 0; w < c->th.size(); w++) th[w] = mbg::window_th(r, w);
-    return MBG_OK;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        const mbg::Results r = results_view(c);
+        if (prob)
+            for (size_t i = 0; i < c->prob.size(); i++) prob[i] = mbg::window_prob(r, i);
+        if (th)
+            for (size_t w = 0; w < c->th.size(); w++) th[w] = mbg::window_th(r, w);
+    });
 }
 
 mbg_status mbg_get_final_states(const mbg_ctx* c, uint64_t* counts) {
@@ -1153,6 +1214,8 @@ extern "C" {
 mbg_status mbg_get_marginals(const mbg_ctx* c, double* out) {
     if (!c || !out) return MBG_ERR_ARG;
     if (!c->have) return MBG_ERR_NOT_BUILT;
+    const mbg_status st = guarded(const_cast<mbg_ctx*>(c), [&] { ensure_prob(c); });
+    if (st != MBG_OK) return st;
     const int W = c->info.windows, K = c->info.readouts;
     std::vector<uint64_t> raw((size_t)W * K, 0);
     const uint64_t S = 1ull << K;
