@@ -71,6 +71,14 @@ def test_fixed_point_heavy_bit_exact(gpu_ctx):
     g, r = _compare(gpu_ctx, net.bnd(), net.cfg(), count=256, n=262144)
 
 
+@pytest.mark.parametrize("n", [1100, 1300, 2000])
+def test_mid_size_networks_bit_exact(gpu_ctx, n):
+    """17-32 state words: register rate sums in groups of two words (even word counts) or LDS sums."""
+    net = synth.random_network(n, 2, 4, seed=n, rates="loguniform", istate="uniform", readouts=16, max_time=20.0,
+                               time_tick=0.5, sample_count=256)
+    _compare(gpu_ctx, net.bnd(), net.cfg(), count=256)
+
+
 def test_block_size_and_grid_independent(gpu_ctx):
     net = synth.config_small()
     m = Model(net.bnd(), net.cfg())

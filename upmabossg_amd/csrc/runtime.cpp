@@ -333,13 +333,19 @@ int env_int(const char* name, int dflt) {
     return e ? std::atoi(e) : dflt;
 }
 
+// Word rate sums live in VGPRs (an unrolled select over at most 16 sums per update) when
+// the model has at most 16 words, or up to 32 words in groups of two (an even word count).
+// Otherwise they take a third LDS array beside x and act.
+int default_wsum_reg(int nw) { return env_int("MBG_TUNE_WSUM_REG", (nw <= 16 || (nw <= 32 && nw % 2 == 0)) ? 1 : 0) ? 1 : 0; }
+
 // Words per register rate-sum group: fewer, wider groups make each sum update cheaper (an
 // unrolled select over the groups) at the cost of scanning more act words per selection.
 int default_sumw(int nw, int wsum_reg) {
     if (!wsum_reg) return 1;
-    int sw = env_int("MBG_TUNE_SUMW", 1);
+    int sw = env_int("MBG_TUNE_SUMW", nw > 16 ? 2 : 1);
     if (sw < 1) sw = 1;
     while (sw > 1 && nw % sw) sw /= 2;
+    while (nw / sw > 16 && nw % (sw * 2) == 0) sw *= 2;
     return sw;
 }
 
@@ -351,7 +357,7 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
     const size_t cu_lds = 160 * 1024;
     const int max_in = std::max(1, cm.max_inputs);
     // word sums in VGPRs pay off while the unrolled select over the words stays short
-    const int wsum_reg = env_int("MBG_TUNE_WSUM_REG", cm.NW <= 16 ? 1 : 0) ? 1 : 0;
+    const int wsum_reg = default_wsum_reg(cm.NW);
     const int tables_env = env_int("MBG_TUNE_TABLES", -1);
     if (!block_req) block_req = env_int("MBG_TUNE_BLOCK", 0);
     Shape best{cm.NW, 64, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, 0, max_in};
@@ -669,7 +675,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             sh.rate32 = 1;
             sh.multi = 1;
             sh.compact = env_int("MBG_TUNE_COMPACT", 1) != 0 ? 1 : 0;
-            sh.wsum_reg = nwc <= 16 ? 1 : 0;
+            sh.wsum_reg = default_wsum_reg(nwc);
             sh.sumw = default_sumw(nwc, sh.wsum_reg);
         }
         SweepClass& C = *sw->classes[it->second];
