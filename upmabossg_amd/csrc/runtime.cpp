@@ -628,6 +628,20 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     c->have = true;
 }
 
+// State-word class of a sweep variant: 1, 2, 4, 8, then multiples of 4 words up to 32, then
+// powers of two.  Lanes of a class hold NWc words, so a tight class keeps more lanes in LDS.
+int word_class(int nw) {
+    if (nw <= 8) {
+        int c = 1;
+        while (c < nw) c *= 2;
+        return c;
+    }
+    if (nw <= 32) return (nw + 3) / 4 * 4;
+    int c = 64;
+    while (c < nw) c *= 2;
+    return c;
+}
+
 // Variant sweep: all variants in one HIP graph launch.
 void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -658,8 +672,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         V->m = m;
         V->n = vs[v].n_trajectories ? vs[v].n_trajectories : (uint64_t)cm.sample_count;
         if (V->n == 0 || V->n > (1ull << 32)) throw Error(MBG_ERR_ARG, "invalid trajectory count");
-        int nwc = 1;
-        while (nwc < cm.NW) nwc *= 2;
+        const int nwc = word_class(cm.NW);
         V->NWc = nwc;
         rts[v] = mbg::make_rate_tables(cm, m->mult_up.empty() ? nullptr : m->mult_up.data(),
                                        m->mult_down.empty() ? nullptr : m->mult_down.data(),
