@@ -50,6 +50,12 @@ def test_small_network_bit_exact(gpu_ctx):
     net = synth.config_small()
     g, r = _compare(gpu_ctx, net.bnd(), net.cfg(), count=10000)
     assert g.transitions > 100000
+    # readout marginals reduced on the device = the window table summed over states with each readout ON
+    K = r.readouts
+    bits = (np.arange(1 << K)[:, None] >> np.arange(K)[None, :]) & 1
+    raw = np.array(r.prob.astype(object) @ bits.astype(object), dtype=np.float64)
+    want = raw / (10000 * float(np.float32(net.time_tick)) * 2.0 ** r.prob_exponent)
+    assert np.allclose(gpu_ctx.marginals(), want, rtol=1e-12, atol=0)
 
 
 @pytest.mark.parametrize("name,count", [("cellcycle", 20000), ("metastasis", 1000), ("sizek", 2000),

@@ -790,11 +790,11 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
 
 // Marginal time each readout is ON per window, from a window table: out[w][j] = sum over the
 // projected states s with bit j set of prob[w][s] (exact 64-bit sums).  One block per window.
-extern "C" __global__ __launch_bounds__(256) void mbg_marginals(const u64* prob, int K, u64* out) {
+__device__ __forceinline__ void marginals_block(const u64* prob, int K, u32 w, u64* out) {
     __shared__ u64 part[256 * 20];
     const u32 tid = threadIdx.x;
     const u64 S = 1ull << K;
-    const u64* row = prob + ((u64)blockIdx.x << K);
+    const u64* row = prob + ((u64)w << K);
     u64 acc[20];
 #pragma unroll
     for (int j = 0; j < 20; j++) acc[j] = 0;
@@ -809,6 +809,24 @@ extern "C" __global__ __launch_bounds__(256) void mbg_marginals(const u64* prob,
     if (tid < (u32)K) {
         u64 sum = 0;
         for (int i = 0; i < 256; i++) sum += part[tid * 256 + i];
-        out[(u64)blockIdx.x * K + tid] = sum;
+        out[(u64)w * K + tid] = sum;
     }
+}
+
+extern "C" __global__ __launch_bounds__(256) void mbg_marginals(const u64* prob, int K, u64* out) {
+    marginals_block(prob, K, blockIdx.x, out);
+}
+
+// The marginals of many window tables (the variants of a sweep) in one launch: block (w, v)
+// reduces window w of job v into out + out_off.
+struct mbg_marg_job {
+    const u64* prob;
+    u64 out_off;
+    int W, K;
+};
+
+extern "C" __global__ __launch_bounds__(256) void mbg_marginals_batch(const mbg_marg_job* jobs, u64* out) {
+    const mbg_marg_job J = jobs[blockIdx.y];
+    if ((int)blockIdx.x >= J.W) return;
+    marginals_block(J.prob, J.K, blockIdx.x, out + J.out_off);
 }

@@ -204,7 +204,8 @@ Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt, bool rat
 
 struct Module {
     hipModule_t mod = nullptr;
-    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr;
+    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr,
+                  marginals_batch = nullptr;
 };
 
 // Device buffer kept by the context across runs; it only grows.
@@ -248,16 +249,23 @@ struct VariantRun {
     int NWc = 0;        // words of its class (state rows are NWc words wide)
     uint64_t n = 0;
     RunBuffers buf;
-    DevBuf<uint64_t> marg;
     Desc D{};
     mbg_run_info info{};
     std::vector<uint64_t> fixed_states, fixed_counts;
-    hipFunction_t marginals = nullptr;
+    std::vector<uint64_t> marg_raw;  // [W][K] readout marginals (exact sums), from the sweep graph
+};
+
+// Host mirror of struct mbg_marg_job in device/mbg_kernel.hip.
+struct MargJob {
+    const uint64_t* prob;
+    uint64_t out_off;
+    int W, K;
 };
 
 struct SweepClass {
     Shape sh{};
-    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr;
+    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr,
+                  marginals_batch = nullptr;
     std::vector<int> members;
     DevBuf<Desc> descs;
     size_t smem = 0;
@@ -268,6 +276,13 @@ struct SweepState {
     std::vector<std::unique_ptr<VariantRun>> vars;
     std::vector<std::unique_ptr<SweepClass>> classes;
     mbg_sweep_info info{};
+};
+
+// Device buffers shared by all variants of a sweep, kept by the context across sweeps.
+struct SweepShared {
+    DevBuf<uint64_t> counters;  // [variant][2]: fixed-point count, transitions
+    DevBuf<MargJob> jobs;
+    DevBuf<uint64_t> marg;      // all variants' marginals, [variant][W][K]
 };
 
 }  // namespace
@@ -282,12 +297,15 @@ struct mbg_ctx {
     RunBuffers buf;
     std::unique_ptr<SweepState> sweep;       // last variant sweep
     std::vector<std::unique_ptr<VariantRun>> variant_pool;  // device buffers reused by later sweeps
+    SweepShared sweep_shared;
     std::vector<hipStream_t> class_streams;  // one per word class of a sweep
     // last run
     bool have = false;
     mbg_run_info info{};
     mbg::PinnedU64 prob;  // window table, pinned for a fast device-to-host copy
     bool prob_on_host = false;  // the window table stays on the device until a getter asks
+    hipFunction_t marg_fn = nullptr;  // mbg_marginals of the last run's module
+    DevBuf<uint64_t> marg;
     std::vector<uint64_t> th, fin, fixed_states, fixed_counts, summ;
 };
 
@@ -321,6 +339,7 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
     hip_check(hipModuleGetFunction(&m.sim, m.mod, sh.multi ? "mbg_simulate_multi" : "mbg_simulate"),
               "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.marginals, m.mod, "mbg_marginals"), "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.marginals_batch, m.mod, "mbg_marginals_batch"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_state, m.mod, "mbg_init_state"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_rates, m.mod, "mbg_init_rates"), "hipModuleGetFunction");
     return c->modules.emplace(sh, m).first->second;
@@ -625,6 +644,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     I.tables_in_lds = sh.tables_lds;
     I.lds_bytes = (int)smem;
     I.blocks_per_cu = blocks_per_cu;
+    c->marg_fn = mod.marginals;
     c->have = true;
 }
 
@@ -721,11 +741,19 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         C.init_state = mod.init_state;
         C.init_rates = mod.init_rates;
         C.marginals = mod.marginals;
+        C.marginals_batch = mod.marginals_batch;
         C.smem = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg);
         int nb = 0;
         if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, C.sim, C.sh.block, C.smem) != hipSuccess || nb < 1) nb = 1;
         C.blocks_per_cu = nb;
     }
+    // shared per-sweep buffers: counters and the marginal jobs of every variant
+    SweepShared& SS = c->sweep_shared;
+    SS.counters.zero((size_t)2 * nv, main);
+    std::vector<MargJob> jobs;
+    std::vector<int> job_of(nv, -1);
+    uint64_t marg_total = 0;
+    int w_max = 0;
     // per variant: tables and buffers
     for (auto& Cp : sw->classes) {
         SweepClass& C = *Cp;
@@ -737,7 +765,6 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             const Blob blob = pack_blob(cm, rt, C.sh.rate32 != 0, C.sh.compact != 0);
             const RunScales rs = run_scales(cm, V.n);
             const uint64_t fp_cap = V.n;
-            V.marginals = C.marginals;
             RunBuffers& B = V.buf;
             B.blob.upload(blob.words, main);
             B.rtab.upload(rt.rtab, main);
@@ -747,9 +774,13 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             B.th.zero((size_t)rs.W, main);
             B.fin.zero((size_t)rs.S, main);
             B.fp.alloc((size_t)fp_cap * V.NWc);
-            B.fpc.zero(1, main);
             B.work.zero(1, main);
-            B.flips.zero(1, main);
+            if (cm.K > 0) {
+                job_of[v] = (int)jobs.size();
+                jobs.push_back(MargJob{B.prob.p, marg_total, rs.W, cm.K});
+                marg_total += (uint64_t)rs.W * (uint64_t)cm.K;
+                w_max = std::max(w_max, rs.W);
+            }
             Desc& D = V.D;
             D = Desc{};
             D.blob = B.blob.p;
@@ -760,10 +791,10 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             D.th = B.th.p;
             D.fin = B.fin.p;
             D.fp_states = B.fp.p;
-            D.fp_count = B.fpc.p;
+            D.fp_count = SS.counters.p + 2 * (size_t)v;
             D.summ = nullptr;
             D.work_counter = B.work.p;
-            D.flip_total = B.flips.p;
+            D.flip_total = SS.counters.p + 2 * (size_t)v + 1;
             D.fp_cap = fp_cap;
             D.summ_cap = 0;
             D.nsteps = rs.nsteps;
@@ -806,6 +837,12 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         hip_check(hipMemcpyAsync(C.descs.p, descs.data(), descs.size() * sizeof(Desc), hipMemcpyHostToDevice, main),
                   "hipMemcpyAsync");
     }
+    if (!jobs.empty()) {
+        SS.jobs.alloc(jobs.size());
+        hip_check(hipMemcpyAsync(SS.jobs.p, jobs.data(), jobs.size() * sizeof(MargJob), hipMemcpyHostToDevice, main),
+                  "hipMemcpyAsync");
+        SS.marg.alloc(marg_total);
+    }
     hip_check(hipStreamSynchronize(main), "sweep set-up");
     // one stream per class, forked from and joined to the main stream inside the graph
     while (c->class_streams.size() < sw->classes.size()) {
@@ -844,6 +881,15 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         hip_check(hipEventRecord(evs[ci + 1], st), "hipEventRecord");
         hip_check(hipStreamWaitEvent(main, evs[ci + 1], 0), "hipStreamWaitEvent");
     }
+    if (!jobs.empty()) {
+        // every variant's readout marginals, reduced from its window table on the device
+        const MargJob* jp = SS.jobs.p;
+        uint64_t* mp = SS.marg.p;
+        void* margs[] = {(void*)&jp, (void*)&mp};
+        hip_check(hipModuleLaunchKernel(sw->classes[0]->marginals_batch, (unsigned)w_max, (unsigned)jobs.size(), 1, 256, 1,
+                                        1, 0, main, margs, nullptr),
+                  "launch mbg_marginals_batch");
+    }
     hip_check(hipStreamEndCapture(main, &graph), "hipStreamEndCapture");
     hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     const auto t1 = std::chrono::steady_clock::now();
@@ -868,11 +914,18 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
     SI.classes = (int)sw->classes.size();
     SI.graph_ms = ms;
     SI.setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    for (auto& Vp : sw->vars) {
-        VariantRun& V = *Vp;
-        uint64_t nfp = 0, flips = 0;
-        hip_check(hipMemcpy(&nfp, V.buf.fpc.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
-        hip_check(hipMemcpy(&flips, V.buf.flips.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    std::vector<uint64_t> counters((size_t)2 * nv), marg(marg_total);
+    hip_check(hipMemcpy(counters.data(), SS.counters.p, counters.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (marg_total) hip_check(hipMemcpy(marg.data(), SS.marg.p, marg_total * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    for (int v = 0; v < nv; v++) {
+        VariantRun& V = *sw->vars[v];
+        const uint64_t nfp = counters[2 * (size_t)v], flips = counters[2 * (size_t)v + 1];
+        V.marg_raw.clear();
+        if (job_of[v] >= 0) {
+            const MargJob& J = jobs[(size_t)job_of[v]];
+            V.marg_raw.assign(marg.begin() + (ptrdiff_t)J.out_off,
+                              marg.begin() + (ptrdiff_t)(J.out_off + (uint64_t)J.W * (uint64_t)J.K));
+        }
         if (nfp > V.D.fp_cap) throw Error(MBG_ERR_LIMIT, "fixed-point buffer overflow");
         std::vector<uint64_t> fps((size_t)nfp * V.NWc);
         if (nfp) hip_check(hipMemcpy(fps.data(), V.buf.fp.p, fps.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
@@ -890,22 +943,6 @@ const VariantRun& sweep_variant(const mbg_ctx* c, int v) {
     if (!c->sweep) throw Error(MBG_ERR_NOT_BUILT, "no variant sweep has run");
     if (v < 0 || v >= (int)c->sweep->vars.size()) throw Error(MBG_ERR_ARG, "variant index out of range");
     return *c->sweep->vars[v];
-}
-
-// Per-readout marginals of a window table held on the device.
-void device_marginals(mbg_ctx* c, const VariantRun& V, hipFunction_t fn, std::vector<uint64_t>& out) {
-    const int W = V.info.windows, K = V.info.readouts;
-    out.assign((size_t)W * K, 0);
-    if (K == 0) return;
-    DevBuf<uint64_t>& M = const_cast<DevBuf<uint64_t>&>(V.marg);
-    M.alloc((size_t)W * K);
-    const void* prob = V.buf.prob.p;
-    void* outp = M.p;
-    int k = K;
-    void* args[] = {(void*)&prob, &k, &outp};
-    hip_check(hipModuleLaunchKernel(fn, (unsigned)W, 1, 1, 256, 1, 1, 0, c->stream, args, nullptr), "launch mbg_marginals");
-    hip_check(hipMemcpyAsync(out.data(), M.p, out.size() * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
-    hip_check(hipStreamSynchronize(c->stream), "mbg_marginals");
 }
 
 void normalise_marginals(const mbg_run_info& I, const std::vector<uint64_t>& raw, double* out) {
@@ -1233,23 +1270,26 @@ extern "C" {
 mbg_status mbg_get_marginals(const mbg_ctx* c, double* out) {
     if (!c || !out) return MBG_ERR_ARG;
     if (!c->have) return MBG_ERR_NOT_BUILT;
-    const mbg_status st = guarded(const_cast<mbg_ctx*>(c), [&] { ensure_prob(c); });
-    if (st != MBG_OK) return st;
-    const int W = c->info.windows, K = c->info.readouts;
-    std::vector<uint64_t> raw((size_t)W * K, 0);
-    const uint64_t S = 1ull << K;
-    for (int w = 0; w < W; w++) {
-        const uint64_t* row = c->prob.data() + (uint64_t)w * S;
-        uint64_t* o = raw.data() + (size_t)w * K;
-        for (uint64_t st = 0; st < S; st++) {
-            const uint64_t v = row[st];
-            if (!v) continue;
-            for (int j = 0; j < K; j++)
-                if ((st >> j) & 1ull) o[j] += v;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        mbg_ctx* cc = const_cast<mbg_ctx*>(c);
+        const int W = c->info.windows, K = c->info.readouts;
+        std::vector<uint64_t> raw((size_t)W * K, 0);
+        if (K > 0) {
+            // reduced on the device from the window table (no transfer of the whole table)
+            hip_check(hipSetDevice(c->device), "hipSetDevice");
+            cc->marg.alloc(raw.size());
+            const void* prob = cc->buf.prob.p;
+            void* outp = cc->marg.p;
+            int k = K;
+            void* args[] = {(void*)&prob, &k, &outp};
+            hip_check(hipModuleLaunchKernel(c->marg_fn, (unsigned)W, 1, 1, 256, 1, 1, 0, c->stream, args, nullptr),
+                      "launch mbg_marginals");
+            hip_check(hipMemcpyAsync(raw.data(), cc->marg.p, raw.size() * 8, hipMemcpyDeviceToHost, c->stream),
+                      "hipMemcpyAsync");
+            hip_check(hipStreamSynchronize(c->stream), "mbg_marginals");
         }
-    }
-    normalise_marginals(c->info, raw, out);
-    return MBG_OK;
+        normalise_marginals(c->info, raw, out);
+    });
 }
 
 mbg_status mbg_run_variants(mbg_ctx* c, const mbg_variant* variants, int count) {
@@ -1286,8 +1326,8 @@ mbg_status mbg_get_variant_marginals(const mbg_ctx* c, int v, double* out) {
     if (!c || !out) return MBG_ERR_ARG;
     return guarded(const_cast<mbg_ctx*>(c), [&] {
         const VariantRun& V = sweep_variant(c, v);
-        std::vector<uint64_t> raw;
-        device_marginals(const_cast<mbg_ctx*>(c), V, V.marginals, raw);
+        std::vector<uint64_t> raw = V.marg_raw;
+        raw.resize((size_t)V.info.windows * (size_t)V.info.readouts, 0);
         normalise_marginals(V.info, raw, out);
     });
 }

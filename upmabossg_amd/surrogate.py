@@ -20,6 +20,7 @@ import json
 import math
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -77,16 +78,21 @@ class Ensemble:
 
     def fit(self, X, Y, epochs=40, batch=256, lr=1e-3, seed=0, stages=4, decay=0.3):
         """Adam over `epochs` epochs in `stages` equal stages, the rate multiplied by `decay` per stage."""
-        losses = []
         per = [epochs // stages + (1 if s < epochs % stages else 0) for s in range(stages)]
-        for k, m in enumerate(self.members):
+
+        def one(k):
+            m = self.members[k]
             m.load_dataset(X, Y)
             loss = None
             for s, e in enumerate(per):
                 if e:
                     loss = m.train(e, batch=batch, lr=lr * decay ** s, seed=seed + 1000 * k + s)
-            losses.append(loss)
-        return losses
+            return loss
+
+        # members are independent (own buffers and stream): train them concurrently so their
+        # small per-step kernels overlap on the device
+        with ThreadPoolExecutor(max_workers=len(self.members)) as pool:
+            return list(pool.map(one, range(len(self.members))))
 
     def predict(self, X):
         """(mean, spread, kernel seconds, wall seconds); spread = per point, the largest standard
@@ -100,7 +106,7 @@ class SweepSurrogate:
     """Screen variants of a base model: the ensemble's mean where its spread is at most
     `threshold`, the simulator's marginals (plain simulation) everywhere else."""
 
-    def __init__(self, ctx, base: Model, members=4, hidden=512, hidden_layers=3, seed=0, max_batch=2048):
+    def __init__(self, ctx, base: Model, members=4, hidden=512, hidden_layers=3, seed=0, max_batch=10240):
         self.ctx, self.base = ctx, base
         self.n = base.num_nodes
         self.ens = None
@@ -166,7 +172,7 @@ def run(n_train=50000, n_test=10000, n_ref=64, t_train=4096, t_ref=262144, t_pla
         log(f"training data: {n_train} variants x {t_train} trajectories, graph {gs:.1f} s, wall {ws:.1f} s")
         report["train"] = {"points": n_train, "trajectories_per_point": t_train, "sim_graph_s": gs, "sim_wall_s": ws}
 
-        ens = Ensemble(X.shape[1], Y.shape[1], members=members, seed=seed, max_batch=2048)
+        ens = Ensemble(X.shape[1], Y.shape[1], members=members, seed=seed, max_batch=10240)
         t1 = time.perf_counter()
         losses = ens.fit(X, Y, epochs=epochs, batch=batch, lr=lr)
         fit_s = time.perf_counter() - t1
