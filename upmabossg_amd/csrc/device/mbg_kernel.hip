@@ -226,11 +226,10 @@ __device__ __forceinline__ u64 rate_from_key(const mbg_desc& D, const char* tb, 
 // words run mbg_init_state first and use the per-(trajectory, word) variant below.
 #define INIT_TRAJ 64
 #if MBG_NW <= 128
-extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
+__device__ __forceinline__ void init_rates_block(const mbg_desc& D, u64 t0) {
     __shared__ u64 xs[MBG_NW * INIT_TRAJ];
     const u32 tid = threadIdx.x;
     const u32 lane = tid & 63u;
-    const u64 t0 = (u64)blockIdx.x * INIT_TRAJ;
     for (u32 idx = tid; idx < MBG_NW * INIT_TRAJ; idx += 256) {
         const u32 w = idx / INIT_TRAJ, l = idx % INIT_TRAJ;
         u64 word = 0;
@@ -330,6 +329,21 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
         }
     }
 }
+
+extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
+    init_rates_block(D, (u64)blockIdx.x * INIT_TRAJ);
+}
+
+#if MBG_MULTI
+// The same pass for every variant of a sweep class in one launch: block (x, v) takes
+// trajectories 64x .. 64x + 63 of variant v.
+extern "C" __global__ __launch_bounds__(256) void mbg_init_rates_multi(const mbg_desc* descs) {
+    const mbg_desc& D = descs[blockIdx.y];
+    const u64 t0 = (u64)blockIdx.x * INIT_TRAJ;
+    if (t0 >= D.batch) return;  // the whole block: it holds no barrier yet
+    init_rates_block(D, t0);
+}
+#endif
 #else
 extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
     const u64 gid = (u64)blockIdx.x * 256u + threadIdx.x;

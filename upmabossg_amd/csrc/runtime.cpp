@@ -205,7 +205,7 @@ Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt, bool rat
 struct Module {
     hipModule_t mod = nullptr;
     hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr,
-                  marginals_batch = nullptr;
+                  marginals_batch = nullptr, init_rates_multi = nullptr;
 };
 
 // Device buffer kept by the context across runs; it only grows.
@@ -265,7 +265,7 @@ struct MargJob {
 struct SweepClass {
     Shape sh{};
     hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr,
-                  marginals_batch = nullptr;
+                  marginals_batch = nullptr, init_rates_multi = nullptr;
     std::vector<int> members;
     DevBuf<Desc> descs;
     size_t smem = 0;
@@ -340,6 +340,8 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
               "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.marginals, m.mod, "mbg_marginals"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.marginals_batch, m.mod, "mbg_marginals_batch"), "hipModuleGetFunction");
+    if (sh.multi && sh.nw <= 128)
+        hip_check(hipModuleGetFunction(&m.init_rates_multi, m.mod, "mbg_init_rates_multi"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_state, m.mod, "mbg_init_state"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_rates, m.mod, "mbg_init_rates"), "hipModuleGetFunction");
     return c->modules.emplace(sh, m).first->second;
@@ -742,6 +744,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         C.init_rates = mod.init_rates;
         C.marginals = mod.marginals;
         C.marginals_batch = mod.marginals_batch;
+        C.init_rates_multi = mod.init_rates_multi;
         C.smem = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg);
         int nb = 0;
         if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, C.sim, C.sh.block, C.smem) != hipSuccess || nb < 1) nb = 1;
@@ -860,13 +863,23 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         SweepClass& C = *sw->classes[ci];
         hipStream_t st = c->class_streams[ci];
         hip_check(hipStreamWaitEvent(st, evs[0], 0), "hipStreamWaitEvent");
-        for (int v : C.members) {
-            Desc& D = sw->vars[v]->D;
-            const uint64_t nwc = (uint64_t)sw->vars[v]->NWc;
-            const uint64_t init_threads = D.batch * nwc;
-            const uint64_t rate_blocks = nwc <= 128 ? (D.batch + 63) / 64 : (init_threads + 255) / 256;
-            if (nwc > 128) launch(C.init_state, (init_threads + 255) / 256, 256, 0, st, D, "launch mbg_init_state");
-            launch(C.init_rates, rate_blocks, 256, 0, st, D, "launch mbg_init_rates");
+        if (C.init_rates_multi) {
+            // initial states of all the class's variants in one launch (64 trajectories per block)
+            uint64_t bx = 1;
+            for (int v : C.members) bx = std::max<uint64_t>(bx, (sw->vars[v]->D.batch + 63) / 64);
+            const mbg_desc_ptr dp = C.descs.p;
+            void* iargs[] = {(void*)&dp};
+            hip_check(hipModuleLaunchKernel(C.init_rates_multi, (unsigned)bx, (unsigned)C.members.size(), 1, 256, 1, 1, 0,
+                                            st, iargs, nullptr),
+                      "launch mbg_init_rates_multi");
+        } else {
+            for (int v : C.members) {
+                Desc& D = sw->vars[v]->D;
+                const uint64_t nwc = (uint64_t)sw->vars[v]->NWc;
+                const uint64_t init_threads = D.batch * nwc;
+                launch(C.init_state, (init_threads + 255) / 256, 256, 0, st, D, "launch mbg_init_state");
+                launch(C.init_rates, (init_threads + 255) / 256, 256, 0, st, D, "launch mbg_init_rates");
+            }
         }
         uint64_t traj = 0;
         for (int v : C.members) traj += sw->vars[v]->n;
