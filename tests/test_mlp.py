@@ -112,3 +112,26 @@ def test_gpu_ensemble_predict_matches_members():
     assert np.max(np.abs(spread - P.std(axis=0).max(axis=1))) < 1e-6
     one, s1, _ = ensemble_predict(members[:1], x)
     assert np.array_equal(one, P[0]) and not s1.any()
+
+
+@pytest.mark.gpu
+def test_gpu_large_tiles_match_oracle():
+    """8,192 rows: the forward and input-gradient GEMMs take the 128 x 128 and 128 x 64 tiles."""
+    m = _mlp(seed=9, n_in=100, hidden=512, layers=2, n_out=512, batch=8192)
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=(8192, 100)).astype(np.float32)
+    y = rng.uniform(size=(8192, 512)).astype(np.float32)
+    pred, _ = m.predict(x)
+    layers = [(W.astype(np.float64), b.astype(np.float64)) for W, b in m.unpack(m.get_params())]
+    xp = np.zeros((8192, m.padded_inputs), np.float32)
+    xp[:, :100] = x
+    want, _ = MO.forward(layers, xp, bf16=True)
+    assert np.max(np.abs(pred - want)) < 2e-3
+    loss, g = m.gradients(x, y)
+    want_loss, want_g = MO.loss_and_grads(layers, xp, y, bf16=True)
+    assert abs(loss - want_loss) <= 1e-4 * want_loss + 1e-7
+    for (gw, gb), (ww, wb) in zip(m.unpack(g), want_g):
+        for a, b in [(gw, ww), (gb, wb)]:
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            cos = float(np.dot(a.ravel(), b.ravel()) / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+            assert rel < 0.02 and cos > 0.9995, (rel, cos)

@@ -26,59 +26,91 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int TM = 64, TN = 64, TK = 32, LDT = TK + 8;
+constexpr int TK = 32, LDT = TK + 8;
 
 // A(m, k) = AK ? A[m * lda + k] : A[k * lda + m];  B(k, n) = BK ? B[n * ldb + k] : B[k * ldb + n].
-template <bool AK, bool BK>
+// Block tile BM x BN (64 or 128 each), four waves in 2 x 2, each wave (BM/2) x (BN/2) of
+// 16 x 16 MFMA tiles.  The next K step's operands are loaded into registers while the
+// current step's MFMAs run from LDS.
+template <int ROWS>
+struct Stage {
+    bf16x8 v[ROWS / 64];
+};
+
+// A ROWS x 32 operand tile: with KCONTIG the k index is contiguous in memory (row-major
+// [row][k]), otherwise the row index is ([k][row]).  Each thread moves ROWS/64 groups of 8.
+template <bool KCONTIG, int ROWS>
+__device__ __forceinline__ void stage_load(Stage<ROWS>& st, const bf16* __restrict__ X, long ld, long r0, int k0, int tid) {
+#pragma unroll
+    for (int p = 0; p < ROWS / 64; p++) {
+        if (KCONTIG) {
+            const int r = p * 64 + (tid >> 2), kk = (tid & 3) * 8;
+            st.v[p] = *(const bf16x8*)(X + (r0 + r) * ld + k0 + kk);
+        } else {
+            const int idx = p * 256 + tid;
+            const int kq = idx / (ROWS / 8), mq = (idx % (ROWS / 8)) * 8;
+            st.v[p] = *(const bf16x8*)(X + (long)(k0 + kq) * ld + r0 + mq);
+        }
+    }
+}
+
+template <bool KCONTIG, int ROWS>
+__device__ __forceinline__ void stage_store(const Stage<ROWS>& st, bf16* __restrict__ S, int tid) {
+#pragma unroll
+    for (int p = 0; p < ROWS / 64; p++) {
+        if (KCONTIG) {
+            const int r = p * 64 + (tid >> 2), kk = (tid & 3) * 8;
+            *(bf16x8*)(S + r * LDT + kk) = st.v[p];
+        } else {
+            const int idx = p * 256 + tid;
+            const int kq = idx / (ROWS / 8), mq = (idx % (ROWS / 8)) * 8;
+#pragma unroll
+            for (int i = 0; i < 8; i++) S[(mq + i) * LDT + kq] = st.v[p][i];
+        }
+    }
+}
+
+template <bool AK, bool BK, int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_bf16(const bf16* __restrict__ A, long lda, const bf16* __restrict__ B,
                                                  long ldb, float* __restrict__ C, long ldc, int K, int accumulate) {
-    __shared__ bf16 As[TM * LDT];  // [m][k]
-    __shared__ bf16 Bs[TN * LDT];  // [n][k]
+    __shared__ bf16 As[BM * LDT];  // [m][k]
+    __shared__ bf16 Bs[BN * LDT];  // [n][k]
+    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const long m0 = (long)blockIdx.y * TM, n0 = (long)blockIdx.x * TN;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    f32x4 acc[2][2];
+    const long m0 = (long)blockIdx.y * BM, n0 = (long)blockIdx.x * BN;
+    const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+    f32x4 acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < FN; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    Stage<BM> sa;
+    Stage<BN> sb;
+    stage_load<AK, BM>(sa, A, lda, m0, 0, tid);
+    stage_load<BK, BN>(sb, B, ldb, n0, 0, tid);
     for (int k0 = 0; k0 < K; k0 += TK) {
-        if (AK) {
-            const int r = tid >> 2, kk = (tid & 3) * 8;
-            const bf16x8 v = *(const bf16x8*)(A + (m0 + r) * lda + k0 + kk);
-            *(bf16x8*)(As + r * LDT + kk) = v;
-        } else {
-            const int kk = tid >> 3, mm = (tid & 7) * 8;
-            const bf16x8 v = *(const bf16x8*)(A + (long)(k0 + kk) * lda + m0 + mm);
-#pragma unroll
-            for (int i = 0; i < 8; i++) As[(mm + i) * LDT + kk] = v[i];
-        }
-        if (BK) {
-            const int r = tid >> 2, kk = (tid & 3) * 8;
-            const bf16x8 v = *(const bf16x8*)(B + (n0 + r) * ldb + k0 + kk);
-            *(bf16x8*)(Bs + r * LDT + kk) = v;
-        } else {
-            const int kk = tid >> 3, nn = (tid & 7) * 8;
-            const bf16x8 v = *(const bf16x8*)(B + (long)(k0 + kk) * ldb + n0 + nn);
-#pragma unroll
-            for (int i = 0; i < 8; i++) Bs[(nn + i) * LDT + kk] = v[i];
-        }
+        stage_store<AK, BM>(sa, As, tid);
+        stage_store<BK, BN>(sb, Bs, tid);
         __syncthreads();
-        bf16x8 a[2], b[2];
+        if (k0 + TK < K) {
+            stage_load<AK, BM>(sa, A, lda, m0, k0 + TK, tid);
+            stage_load<BK, BN>(sb, B, ldb, n0, k0 + TK, tid);
+        }
+        bf16x8 a[FM], b[FN];
 #pragma unroll
-        for (int i = 0; i < 2; i++) a[i] = *(const bf16x8*)(As + (wm + i * 16 + (lane & 15)) * LDT + (lane >> 4) * 8);
+        for (int i = 0; i < FM; i++) a[i] = *(const bf16x8*)(As + (wm + i * 16 + (lane & 15)) * LDT + (lane >> 4) * 8);
 #pragma unroll
-        for (int j = 0; j < 2; j++) b[j] = *(const bf16x8*)(Bs + (wn + j * 16 + (lane & 15)) * LDT + (lane >> 4) * 8);
+        for (int j = 0; j < FN; j++) b[j] = *(const bf16x8*)(Bs + (wn + j * 16 + (lane & 15)) * LDT + (lane >> 4) * 8);
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < FM; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < FN; j++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const long row = m0 + wm + i * 16 + (lane >> 4) * 4 + q;
@@ -264,14 +296,25 @@ struct mbg_mlp {
     size_t nset = 0;
     long step = 0;
 
+    template <int BM, int BN>
+    void launch_gemm(const bf16* A, long lda, bool ak, const bf16* B, long ldb, bool bk, float* C, long ldc, long M,
+                     long N, long K, bool acc) {
+        dim3 grid((unsigned)(N / BN), (unsigned)(M / BM));
+        if (ak && bk) gemm_bf16<true, true, BM, BN><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
+        else if (ak && !bk) gemm_bf16<true, false, BM, BN><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
+        else if (!ak && bk) gemm_bf16<false, true, BM, BN><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
+        else gemm_bf16<false, false, BM, BN><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
+    }
+
     void gemm(const bf16* A, long lda, bool ak, const bf16* B, long ldb, bool bk, float* C, long ldc, long M, long N,
               long K, bool acc) {
-        if (M % TM || N % TN || K % TK) throw Error(3, "GEMM dimensions must be multiples of 64x64x32");
-        dim3 grid((unsigned)(N / TN), (unsigned)(M / TM));
-        if (ak && bk) gemm_bf16<true, true><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
-        else if (ak && !bk) gemm_bf16<true, false><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
-        else if (!ak && bk) gemm_bf16<false, true><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
-        else gemm_bf16<false, false><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
+        if (M % 64 || N % 64 || K % TK) throw Error(3, "GEMM dimensions must be multiples of 64x64x32");
+        // the largest tile that still gives every CU a block
+        const bool bm = M % 128 == 0 && (M / 128) * (N / 64) >= 256;
+        const bool bn = N % 128 == 0 && (M / (bm ? 128 : 64)) * (N / 128) >= 256;
+        if (bm && bn) launch_gemm<128, 128>(A, lda, ak, B, ldb, bk, C, ldc, M, N, K, acc);
+        else if (bm) launch_gemm<128, 64>(A, lda, ak, B, ldb, bk, C, ldc, M, N, K, acc);
+        else launch_gemm<64, 64>(A, lda, ak, B, ldb, bk, C, ldc, M, N, K, acc);
         check(hipGetLastError(), "gemm launch");
     }
 
