@@ -134,6 +134,29 @@ EDGE_MODELS = {
 }
 
 
+def _wide_fanin_model():
+    # node Z reads 18 inputs (truth table of 2^19 keys: the wide table layout); the inputs toggle
+    names = [f"x{i}" for i in range(18)]
+    bnd = "".join(f"node {n} {{ logic = !{n}; rate_up = @logic ? {0.5 + 0.1 * i:.1f} : 0; "
+                  f"rate_down = @logic ? 0 : {1.5 - 0.05 * i:.2f}; }}\n" for i, n in enumerate(names))
+    expr = "((" + " & ".join(names[:6]) + ") | (" + " & ".join(names[6:12]) + ")) ^ (" + " | ".join(names[12:]) + ")"
+    bnd += f"node Z {{ logic = {expr}; rate_up = @logic ? 2 : 0; rate_down = @logic ? 0 : 3; }}\n"
+    cfg = "".join(f"{n}.is_internal = 1;\n" for n in names[4:]) + \
+        "max_time = 4; time_tick = 0.25; sample_count = 3000; seed_pseudorandom = 5;"
+    return bnd, cfg
+
+
+def _twenty_readouts_model():
+    # 20 non-internal nodes: 2^20 projected states per window
+    bnd = "".join(f"node n{i} {{ logic = n{(i + 1) % 20} | !n{(i + 7) % 20}; rate_up = @logic ? 1.{i % 10} : 0; "
+                  f"rate_down = @logic ? 0 : 0.{i % 9 + 1}; }}\n" for i in range(20))
+    return bnd, "max_time = 3; time_tick = 0.5; sample_count = 4000;"
+
+
+EDGE_MODELS["wide_fanin"] = _wide_fanin_model()
+EDGE_MODELS["twenty_readouts"] = _twenty_readouts_model()
+
+
 @pytest.mark.parametrize("name", sorted(EDGE_MODELS))
 def test_edge_models_bit_exact(gpu_ctx, name):
     b, c = EDGE_MODELS[name]

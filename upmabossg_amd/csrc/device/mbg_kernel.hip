@@ -428,6 +428,12 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
     const u32 key0 = (u32)D.seed, key1 = (u32)(D.seed >> 32);
 
     bool have = false, exhausted = false;
+    // Step parity is kept equal across the wave: a trajectory starts only on an even loop
+    // iteration (a lane refilled on an odd one holds for one iteration), so every active lane
+    // has the same k & 1 and the Philox call (one per two steps) runs on every other
+    // iteration only, instead of on every iteration for a mix of parities.
+    u32 iter = 0;
+    bool hold = false;
     u64 id = 0, total = 0, k = 0, acc_p = 0, acc_th = 0, flips = 0;
     u64 ro_R = 0, ro_S = 0;  // readout rate sum and entropy-term sum (exact)
 #if MBG_WSUM_REG
@@ -446,6 +452,8 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #endif
 
     for (;;) {
+        const bool odd_iter = (iter & 1u) != 0u;
+        iter++;
         // ---- refill idle lanes from the work queue (one atomic per wave) ----
         const bool need = !have && !exhausted;
         const u64 mask = __ballot(need);
@@ -458,6 +466,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 const u64 g = base + (u64)__popcll(mask & ((1ull << lid) - 1ull));
                 if (g < D.batch) {
                     have = true;
+                    hold = odd_iter;
                     id = D.batch_first + g;
                     const u64* st = D.stage + g * 3 * MBG_NW;
                     total = 0;
@@ -511,6 +520,10 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         }
         if (__ballot(have) == 0ull) break;
         if (!have) continue;
+        if (hold) {
+            hold = false;
+            continue;
+        }
 
         // ---- one Gillespie step (src/jit_kernels/simulation.cu:106-148) ----
         double t1, th_use = 0.0;
