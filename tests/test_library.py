@@ -133,3 +133,30 @@ def test_cli_usage_and_parse_errors(tmp_path):
     assert r.returncode == 1 and "syntax error" in r.stderr
     r = subprocess.run([CLI_PATH, str(tmp_path / "missing.bnd"), str(cfg)], capture_output=True, text=True)
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+def _gpu_visible():
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        n = ctypes.c_int(0)
+        return hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+    except OSError:
+        return False
+
+
+@pytest.mark.skipif(_gpu_visible(), reason="a GPU is visible: the no-GPU error path cannot be exercised")
+def test_no_cpu_fallback_without_a_gpu(tmp_path):
+    """Without a GPU the simulator and the surrogate network refuse to run (status 4, HIP error)
+    instead of computing anything on the CPU."""
+    from upmabossg_amd import Context
+    from upmabossg_amd.mlp import Mlp, MlpError
+
+    with pytest.raises(MbgError) as e:
+        Context(0)
+    assert e.value.status == 4
+    with pytest.raises(MlpError):
+        Mlp(10, 64, 1, 64, max_batch=64)
+    net = synth.config_small()
+    bnd, cfg = net.write(str(tmp_path / "m"))
+    out = subprocess.run([CLI_PATH, bnd, cfg], capture_output=True, text=True)
+    assert out.returncode == 1 and "HIP" in (out.stderr + out.stdout)
