@@ -1,0 +1,111 @@
+"""The GPU simulator against the reference's own result fixtures (test/*.res, MaBoSSG stdout
+format), each model run at its own sample_count, seed and clock.  The fixtures come from the
+reference's curand streams, so agreement is statistical: binomial z-scores per entry (final
+points, fixed points, every window's projected-state probabilities), both runs' sampling
+noise combined.
+
+- metastasis (1,000 trajectories) and sizek (100,000): every entry within 4 sigma.
+- Montagud2022_Prostate_Cancer (1,000,000 trajectories, 500 windows): the same 2,264 fixed
+  points; at least 97 % of the window entries within 4 sigma.  The rest carry the reference's
+  batch-boundary extra sojourn (every 99 transitions its kernel re-draws the waiting time of
+  an unflipped state, src/jit_kernels/simulation.cu:104-143), which this simulator does not
+  reproduce: uninterrupted Gillespie.
+- cellcycle (1,000,000 trajectories, discrete time): every window entry within 4 sigma.  The
+  fixture's "final points" block disagrees with its own last window (CycD 0.519 vs 0.428; the
+  final state is the state of the last record, src/jit_kernels/final_states.cu:9-21, so the two
+  must agree); it comes from an older build.  Here the final states match the fixture's last
+  window.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from tests.conftest import FIXTURES, model_text
+from upmabossg_amd import Model
+
+pytestmark = pytest.mark.gpu
+
+
+def _parse_res(path):
+    out = {"final": {}, "fixed": {}, "windows": []}
+    sec = None
+    with open(path, encoding="utf-8") as f:
+        for ln in f.read().splitlines():
+            if ln.startswith("final points:"):
+                sec = "final"
+            elif ln.startswith("fixed points:"):
+                sec = "fixed"
+            elif ln.startswith("window ("):
+                sec = "win"
+                out["windows"].append({})
+            elif ln and not ln.startswith("entropy") and not ln.startswith("transition"):
+                p, st = ln.split(" ", 1)
+                key = frozenset() if st == "<nil>" else frozenset(st.split("-"))
+                if sec == "win":
+                    out["windows"][-1][key] = float(p)
+                elif sec:
+                    out[sec][key] = float(p)
+    return out
+
+
+def _z_scores(mine, theirs, n, n_ref):
+    z = {}
+    for k in set(mine) | set(theirs):
+        a, e = mine.get(k, 0.0), theirs.get(k, 0.0)
+        var = max(e * (1 - e), 1.0 / n_ref) / n_ref + max(a * (1 - a), 1.0 / n) / n
+        z[k] = (a - e) / math.sqrt(var)
+    return z
+
+
+def _run(gpu_ctx, name):
+    b, c = model_text(name)
+    m = Model(b, c)
+    n = int(m.constant("sample_count"))
+    print(f"case: {name} n={n}", flush=True)
+    r = gpu_ctx.run(m)
+    names, internal = m.names(), m.internal()
+    ro = [i for i in range(len(names)) if not internal[i]]
+
+    def proj(s):
+        return frozenset(names[ro[j]] for j in range(len(ro)) if (s >> j) & 1)
+
+    windows = [{proj(s): float(r.probtraj[w][s]) for s in np.nonzero(r.probtraj[w])[0]} for w in range(r.windows)]
+    final = {proj(s): float(c_) / n for s, c_ in enumerate(r.final_states) if c_}
+    fixed = {}
+    for st, c_ in zip(r.fixed_states, r.fixed_counts):
+        fixed[frozenset(names[i] for i in range(len(names)) if (int(st[i >> 6]) >> (i & 63)) & 1)] = float(c_) / n
+    return n, windows, final, fixed, _parse_res(os.path.join(FIXTURES, name + ".res"))
+
+
+@pytest.mark.parametrize("name", ["metastasis", "sizek"])
+def test_every_entry_within_4_sigma(gpu_ctx, name):
+    n, windows, final, fixed, ref = _run(gpu_ctx, name)
+    assert len(windows) == len(ref["windows"])
+    zs = list(_z_scores(final, ref["final"], n, n).values()) + list(_z_scores(fixed, ref["fixed"], n, n).values())
+    for mine, theirs in zip(windows, ref["windows"]):
+        zs += list(_z_scores(mine, theirs, n, n).values())
+    assert max(abs(z) for z in zs) <= 4.0, max(abs(z) for z in zs)
+
+
+def test_montagud_fixed_points_and_windows(gpu_ctx):
+    n, windows, final, fixed, ref = _run(gpu_ctx, "Montagud2022_Prostate_Cancer")
+    assert len(fixed) == len(ref["fixed"]) == 2264
+    assert len(windows) == len(ref["windows"]) == 500
+    zs = []
+    for mine, theirs in zip(windows, ref["windows"]):
+        zs += list(_z_scores(mine, theirs, n, n).values())
+    zs = np.abs(np.array(zs))
+    assert (zs <= 4.0).mean() >= 0.97 and zs.max() < 8.0, ((zs <= 4.0).mean(), zs.max())
+    fz = np.abs(np.array(list(_z_scores(fixed, ref["fixed"], n, n).values())))
+    assert (fz <= 4.0).mean() >= 0.97, (fz <= 4.0).mean()
+
+
+def test_cellcycle_windows_and_final_states(gpu_ctx):
+    n, windows, final, fixed, ref = _run(gpu_ctx, "cellcycle")
+    assert len(windows) == len(ref["windows"]) == 25 and not fixed and not ref["fixed"]
+    for mine, theirs in zip(windows, ref["windows"]):
+        assert max(abs(z) for z in _z_scores(mine, theirs, n, n).values()) <= 4.0
+    # final states = the state of the last record: consistent with the fixture's last window
+    assert max(abs(z) for z in _z_scores(final, ref["windows"][-1], n, n).values()) <= 4.0
