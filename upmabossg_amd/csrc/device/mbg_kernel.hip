@@ -15,6 +15,7 @@
 //   MBG_MULTI       1: emit the variant-sweep kernel instead of the single-model one
 //   MBG_SUMW        64-bit words per rate-sum group (register sums only; divides MBG_NW)
 //   MBG_COMPACT     1: 8-byte node records and 16-bit dependents (small models, no generic rates)
+//   MBG_WSMAD       1: every rate is below 2^31 (register sum updates by multiply-add)
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -65,6 +66,9 @@ struct mbg_node {
 
 #ifndef MBG_COMPACT
 #define MBG_COMPACT 0
+#endif
+#ifndef MBG_WSMAD
+#define MBG_WSMAD 0
 #endif
 
 // A node record decoded into registers (either table layout).
@@ -166,12 +170,26 @@ __device__ __forceinline__ u32 dep_entry(u32 v) { return (v & 0xfffu) | ((v >> 1
 // over the MBG_NW / MBG_SUMW groups
 #define NSUM (MBG_NW / MBG_SUMW)
 #define LANE_S(g) ws[g]
+#if MBG_WSMAD
+// every rate change fits an i32: ws[i] += [i == g] * d as one 32 x 32 + 64 multiply-add
+#define WS_ADD(w, d)                                                       \
+    do {                                                                   \
+        const u32 oh_ = 1u << ((w) / MBG_SUMW);                            \
+        const long long d_ = (long long)(int)(u32)(d);                     \
+        _Pragma("unroll") for (int i_ = 0; i_ < NSUM; i_++) {              \
+            u32 f_ = (oh_ >> i_) & 1u;                                     \
+            asm volatile("" : "+v"(f_)); /* keep the multiply-add */       \
+            ws[i_] = (u64)((long long)ws[i_] + (long long)(int)f_ * d_);   \
+        }                                                                  \
+    } while (0)
+#else
 #define WS_ADD(w, d)                                                       \
     do {                                                                   \
         const u32 g_ = (w) / MBG_SUMW;                                     \
         const u64 d_ = (d);                                                \
         _Pragma("unroll") for (int i_ = 0; i_ < NSUM; i_++) ws[i_] += g_ == (u32)i_ ? d_ : 0ull; \
     } while (0)
+#endif
 #else
 #undef MBG_SUMW
 #define MBG_SUMW 1

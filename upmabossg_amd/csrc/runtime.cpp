@@ -96,11 +96,12 @@ using mbg_desc_ptr = const void*;
 
 struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 4, scan = 4, wsum_reg = 0, rate32 = 0,
-        multi = 0, sumw = 1, compact = 0;
+        multi = 0, sumw = 1, compact = 0, wsmad = 0;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
-                        sumw, compact) < std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in,
-                                                  o.dep_batch, o.scan, o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact);
+                        sumw, compact, wsmad) < std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds,
+                                                         o.max_in, o.dep_batch, o.scan, o.wsum_reg, o.rate32, o.multi,
+                                                         o.sumw, o.compact, o.wsmad);
     }
 };
 
@@ -122,6 +123,7 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_MULTI " << sh.multi << "\n";
     os << "#define MBG_SUMW " << sh.sumw << "\n";
     os << "#define MBG_COMPACT " << sh.compact << "\n";
+    os << "#define MBG_WSMAD " << sh.wsmad << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -136,6 +138,16 @@ struct Blob {
 bool rates_fit_u32(const mbg::RateTables& rt) {
     for (uint64_t r : rt.rates)
         if (r >> 32) return false;
+    return true;
+}
+
+// Every rate (fast and generic tables) below 2^31: any change of one rate fits an i32, so the
+// register rate sums can be updated with one 32 x 32 + 64 multiply-add each.
+bool rates_fit_i31(const mbg::RateTables& rt) {
+    for (uint64_t r : rt.rates)
+        if (r >> 31) return false;
+    for (uint64_t r : rt.rtab)
+        if (r >> 31) return false;
     return true;
 }
 
@@ -481,6 +493,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     const Blob blob = pack_blob(cm, rt, rate32, compact);
     Shape sh = choose_shape(c, cm, blob.lay.bytes, p->block, p->trace, rate32 ? 1 : 0);
     sh.compact = compact ? 1 : 0;
+    sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
     const Module& mod = get_module(c, cm, sh);
     const size_t smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0);
     int blocks_per_cu = 0;
@@ -710,6 +723,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             sh.rate32 = 1;
             sh.multi = 1;
             sh.compact = env_int("MBG_TUNE_COMPACT", 1) != 0 ? 1 : 0;
+            sh.wsmad = env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
             sh.wsum_reg = default_wsum_reg(nwc);
             sh.sumw = default_sumw(nwc, sh.wsum_reg);
         }
@@ -719,6 +733,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         C.sh.max_in = std::max(C.sh.max_in, cm.max_inputs);
         C.sh.rate32 &= fits32[v];
         C.sh.compact &= compact_fits(cm) ? 1 : 0;
+        C.sh.wsmad &= rates_fit_i31(rts[v]) ? 1 : 0;
         sw->vars.push_back(std::move(V));
     }
     // per class: block size for the lanes' LDS, module, occupancy
@@ -1180,6 +1195,7 @@ mbg_status mbg_compile(mbg_ctx* c, const mbg_model* m, int trace) {
         const bool compact = compact_fits(cm) && env_int("MBG_TUNE_COMPACT", 1) != 0;
         Shape sh = choose_shape(c, cm, pack_blob(cm, rt, r32, compact).lay.bytes, 0, trace, r32 ? 1 : 0);
         sh.compact = compact ? 1 : 0;
+        sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
         get_module(c, cm, sh);
     });
 }
