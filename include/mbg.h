@@ -92,6 +92,9 @@ typedef struct mbg_run_params {
     int trace;               /* 1: record per-trajectory summaries (count entries) */
     int block;               /* threads per block (multiple of 64); 0 = default */
     int grid;                /* blocks; 0 = fill the device */
+    int compat_batch_resume; /* 1: reproduce the reference's 100-record launches (no flip after
+                                every 99th step, a fresh waiting time for the same state next;
+                                src/jit_kernels/simulation.cu:104-147); 0: uninterrupted Gillespie */
 } mbg_run_params;
 
 /* Simulates and accumulates the statistics (replaces simulation_runner::run_simulation,

@@ -940,6 +940,13 @@ static void simulate_range(const orc_model& m, const orc_params& p, uint64_t b, 
             }
             t = t1;
             if (t >= maxt) break;
+            // the reference runs at most 100 records per launch: the 99th step of a launch is
+            // recorded but not followed by a flip, and the next launch draws a fresh waiting
+            // time for the same state (src/jit_kernels/simulation.cu:104-147)
+            if (p.compat_batch_resume && (k + 1) % 99 == 0) {
+                k++;
+                continue;
+            }
             // flip selection (simulation.cu:9-21) on exact fixed-point prefix sums
             uint64_t th32 = total >> 32, tl32 = total & 0xffffffffull;
             uint64_t target = (uint64_t)d2 * th32 + (((uint64_t)d2 * tl32) >> 32);

@@ -64,6 +64,8 @@ typedef struct orc_params {
     int threads;               /* worker threads (>= 1) */
     int record_fixed;          /* collect fixed-point states */
     int record_summaries;      /* fill per-trajectory summaries */
+    int compat_batch_resume;   /* 1: no flip after every 99th step, as the reference's 100-record
+                                  launches do (src/jit_kernels/simulation.cu:104-147) */
 } orc_params;
 
 typedef struct orc_summary {

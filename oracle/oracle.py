@@ -30,6 +30,7 @@ class _Params(C.Structure):
         ("threads", C.c_int),
         ("record_fixed", C.c_int),
         ("record_summaries", C.c_int),
+        ("compat_batch_resume", C.c_int),
     ]
 
 
@@ -199,7 +200,7 @@ class OracleModel:
         return lib().orc_node_rate_fixed(self._h, i, w.ctypes.data_as(C.POINTER(C.c_uint64)))
 
     def run(self, count=None, first_id=0, n_trajectories=None, seed=None, variant_id=0, threads=1,
-            record_fixed=True, record_summaries=False) -> OracleResult:
+            record_fixed=True, record_summaries=False, compat_batch_resume=False) -> OracleResult:
         L = lib()
         sc = int(self.constant("sample_count"))
         if count is None:
@@ -209,7 +210,7 @@ class OracleModel:
         if seed is None:
             seed = int(self.constant("seed_pseudorandom"))
         p = _Params(n_trajectories, first_id, count, seed, variant_id, threads, int(record_fixed),
-                    int(record_summaries))
+                    int(record_summaries), int(compat_batch_resume))
         err = C.create_string_buffer(4096)
         r = L.orc_run(self._h, C.byref(p), err, 4096)
         if not r:

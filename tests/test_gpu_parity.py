@@ -85,6 +85,23 @@ def test_mid_size_networks_bit_exact(gpu_ctx, n):
     _compare(gpu_ctx, net.bnd(), net.cfg(), count=256)
 
 
+def test_batch_resume_compat_bit_exact(gpu_ctx):
+    """The reference's 100-record launches (no flip after every 99th step) on the GPU and in
+    the oracle: identical; and different from uninterrupted Gillespie."""
+    net = synth.random_network(60, 2, 3, 5, rates="loguniform", istate="uniform", readouts=8, max_time=30.0,
+                               time_tick=1.0, sample_count=2000)
+    m, o = Model(net.bnd(), net.cfg()), OracleModel(net.bnd(), net.cfg())
+    print("case: compat_batch_resume 60 nodes x 2000", flush=True)
+    g = gpu_ctx.run(m, trace=True, compat_batch_resume=True)
+    r = o.run(count=2000, threads=16, record_summaries=True, compat_batch_resume=True)
+    assert np.array_equal(g.summaries[:, 0], r.summaries["flips"])
+    assert np.array_equal(g.summaries[:, 1], r.summaries["flip_hash"])
+    assert np.array_equal(g.prob_raw, r.prob) and np.array_equal(g.th_raw, r.th)
+    assert np.array_equal(g.final_states, r.final)
+    plain = gpu_ctx.run(m, trace=True)
+    assert not np.array_equal(plain.summaries[:, 1], g.summaries[:, 1])
+
+
 def test_block_size_and_grid_independent(gpu_ctx):
     net = synth.config_small()
     m = Model(net.bnd(), net.cfg())

@@ -50,6 +50,7 @@ class _RunParams(C.Structure):
         ("trace", C.c_int),
         ("block", C.c_int),
         ("grid", C.c_int),
+        ("compat_batch_resume", C.c_int),
     ]
 
 
@@ -288,11 +289,11 @@ class Context:
         self._check(lib().mbg_compile(self._h, model.handle, int(trace)))
 
     def run(self, model: Model, n_trajectories=0, count=0, first_id=0, seed=None, variant_id=0, trace=False,
-            block=0, grid=0, fetch=True) -> RunResult | None:
+            block=0, grid=0, fetch=True, compat_batch_resume=False) -> RunResult | None:
         if count and not n_trajectories:
             n_trajectories = count  # a partial run is normalised by its own count unless told otherwise
         p = _RunParams(n_trajectories, first_id, count, 0 if seed is None else seed, int(seed is None),
-                       variant_id, int(trace), block, grid)
+                       variant_id, int(trace), block, grid, int(compat_batch_resume))
         self._check(lib().mbg_run(self._h, model.handle, C.byref(p)))
         return self.result(trace) if fetch else None
 

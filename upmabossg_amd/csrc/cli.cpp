@@ -5,6 +5,7 @@
 // fixed points and windows to stdout.  Exit status 1 on usage, parse, limit or device errors.
 // Runs entirely through the C ABI of libmbg.so (include/mbg.h).
 #include <cstdio>
+#include <cstdlib>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -40,6 +41,10 @@ int main(int argc, char** argv) {
     mbg_run_params p = {};
     p.n_trajectories = (uint64_t)(int)sample_count;  // src/main.cpp:131 (float constant to int)
     p.use_model_seed = 1;
+    // MBG_BATCH_RESUME=1 reproduces the reference's 100-record launches (a fresh waiting time
+    // for the same state after every 99th step) instead of uninterrupted Gillespie
+    const char* br = std::getenv("MBG_BATCH_RESUME");
+    p.compat_batch_resume = br && std::atoi(br) != 0 ? 1 : 0;
     int rc = 0;
     if (mbg_run(ctx, model, &p) != MBG_OK) {
         std::cerr << mbg_last_error(ctx) << std::endl;

@@ -139,6 +139,8 @@ struct mbg_desc {
     mbg_layout lay;
     u32 variant_id;
     int N, K, W, FP, FT, sigma;
+    int compat;               // 1: no flip after every 99th step (the reference's 100-record launches)
+    int pad_;
 };
 
 #if MBG_COMPACT
@@ -651,6 +653,13 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             }
 #endif
             have = false;
+            continue;
+        }
+        // compatibility with the reference's 100-record launches: the 99th step of a launch is
+        // recorded but not followed by a flip, and the next launch draws a fresh waiting time for
+        // the same state (src/jit_kernels/simulation.cu:104-147)
+        if (D.compat && (k + 1ull) % 99ull == 0ull) {
+            k++;
             continue;
         }
 

@@ -90,6 +90,7 @@ struct Desc {
     Layout lay;
     uint32_t variant_id;
     int N, K, W, FP, FT, sigma;
+    int compat, pad_;
 };
 
 using mbg_desc_ptr = const void*;
@@ -577,6 +578,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     D.FP = FP;
     D.FT = FT;
     D.sigma = rt.sigma;
+    D.compat = p->compat_batch_resume ? 1 : 0;
 
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
