@@ -5,7 +5,8 @@
 Workload ("large"): a synthetic random Boolean network of 1,000 nodes (in-degree 2-4, random
 AND/OR/NOT formulas, rate_up/rate_down log-uniform in [0.1, 10] FP32, istate U[0, 1], 16
 readout nodes), 1,048,576 trajectories, max_time 50, time_tick 0.5 (100 windows).  One step is
-one full run: every trajectory simulated to max_time and the window, final-state and
+one full run (after an untimed autotune of the launch shape, like the hipRTC compile): every
+trajectory simulated to max_time and the window, final-state and
 fixed-point tables accumulated; the small tables (final states, fixed points, counters) return
 to the host, the 52 MB window table stays in device memory (the library copies it on demand).
 config.transitions_per_s_with_host_copy repeats the steps with the window table copied to
@@ -156,6 +157,8 @@ def main():
     ap.add_argument("--nodes", type=int, default=1000)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the untuned launch shape (and the MBG_TUNE_* environment knobs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (reference kernel for the HBM counters)")
     args = ap.parse_args()
@@ -175,6 +178,9 @@ def main():
     if args.calibrate_copy:
         copy_calibration()
     with Context(0) as ctx:
+        # set-up, untimed: measured launch shape for this model (the hipRTC compiles happen here)
+        tune = None if args.no_autotune else ctx.autotune(model)
+        print(f"autotune: {tune}", file=sys.stderr, flush=True)
         ctx.compile(model)
         for _ in range(args.warmup):
             ctx.run(model, fetch=False)
@@ -230,6 +236,8 @@ def main():
             "trajectories_per_s": info.simulated * args.steps / wall,
             "kernel_ms": kern_s * 1e3,
             "transitions_per_s_with_host_copy": transitions / wall_host,
+            "launch_shape": ({k: tune[k] for k in ("block", "tables_in_lds", "wsum_reg", "compact", "dep_batch",
+                                                   "scan", "wsmad")} if tune else "untuned"),
         },
         "roofline": {
             "bound": "valu",

@@ -102,6 +102,24 @@ def test_batch_resume_compat_bit_exact(gpu_ctx):
     assert not np.array_equal(plain.summaries[:, 1], g.summaries[:, 1])
 
 
+def test_autotuned_shape_gives_identical_results(gpu_ctx):
+    """mbg_autotune measures candidate launch shapes; whichever it keeps, the results are the
+    same integers as with the default shape."""
+    from upmabossg_amd import Context
+
+    net = synth.config_large()
+    m = Model(net.bnd(), net.cfg())
+    print("case: autotune 1000 nodes", flush=True)
+    with Context(0) as ctx:
+        ref = ctx.run(m, count=8192, n_trajectories=8192, trace=True)
+        info = ctx.autotune(m, sample=32768, trace=True)
+        assert info["trials"] >= 4 and info["best_ms"] <= info["default_ms"]
+        tuned = ctx.run(m, count=8192, n_trajectories=8192, trace=True)
+        assert tuned.transitions == ref.transitions
+        assert np.array_equal(tuned.summaries, ref.summaries)
+        assert np.array_equal(tuned.prob_raw, ref.prob_raw) and np.array_equal(tuned.final_states, ref.final_states)
+
+
 def test_block_size_and_grid_independent(gpu_ctx):
     net = synth.config_small()
     m = Model(net.bnd(), net.cfg())

@@ -39,6 +39,12 @@ class MbgError(RuntimeError):
         self.status = status
 
 
+class _TuneInfo(C.Structure):
+    _fields_ = [("block", C.c_int), ("tables_in_lds", C.c_int), ("wsum_reg", C.c_int), ("compact", C.c_int),
+                ("dep_batch", C.c_int), ("scan", C.c_int), ("wsmad", C.c_int), ("trials", C.c_int),
+                ("sample", C.c_uint64), ("best_ms", C.c_double), ("default_ms", C.c_double)]
+
+
 class _RunParams(C.Structure):
     _fields_ = [
         ("n_trajectories", C.c_uint64),
@@ -112,6 +118,7 @@ def lib():
         "mbg_last_error": (cp, [vp]),
         "mbg_compile": (C.c_int, [vp, vp, C.c_int]),
         "mbg_run": (C.c_int, [vp, vp, C.POINTER(_RunParams)]),
+        "mbg_autotune": (C.c_int, [vp, vp, C.c_uint64, C.c_int, C.POINTER(_TuneInfo)]),
         "mbg_get_run_info": (C.c_int, [vp, C.POINTER(_RunInfo)]),
         "mbg_get_probtraj_raw": (C.c_int, [vp, u64p, u64p]),
         "mbg_get_probtraj": (C.c_int, [vp, f64p, f64p]),
@@ -296,6 +303,13 @@ class Context:
                        variant_id, int(trace), block, grid, int(compat_batch_resume))
         self._check(lib().mbg_run(self._h, model.handle, C.byref(p)))
         return self.result(trace) if fetch else None
+
+    def autotune(self, model: Model, sample=0, trace=False) -> dict:
+        """Measure candidate launch shapes on `sample` trajectories and keep the fastest for
+        later runs of models of this structure (mbg_autotune)."""
+        t = _TuneInfo()
+        self._check(lib().mbg_autotune(self._h, model.handle, sample, int(trace), C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _TuneInfo._fields_}
 
     def info(self) -> _RunInfo:
         i = _RunInfo()

@@ -67,7 +67,7 @@ def embed_sources() -> str:
 
 
 def kernel_source(nw: int, block: int = 64, discrete: int = 0, trace: int = 0, generic: int = 0,
-                  tables_lds: int = 1, max_in: int = 4, dep_batch: int = 4, scan: int = 4, wsum_reg: int = 0,
+                  tables_lds: int = 1, max_in: int = 4, dep_batch: int = 2, scan: int = 4, wsum_reg: int = 0,
                   rate32: int = 0, multi: int = 0, sumw: int = 1, compact: int = 0) -> str:
     with open(os.path.join(DEV, "mbg_math.h")) as f:
         math_src = f.read()

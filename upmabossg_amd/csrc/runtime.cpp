@@ -12,6 +12,7 @@
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <set>
 #include <memory>
 #include <numeric>
 #include <sstream>
@@ -96,7 +97,7 @@ struct Desc {
 using mbg_desc_ptr = const void*;
 
 struct Shape {
-    int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 4, scan = 4, wsum_reg = 0, rate32 = 0,
+    int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 2, scan = 4, wsum_reg = 0, rate32 = 0,
         multi = 0, sumw = 1, compact = 0, wsmad = 0;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
@@ -307,6 +308,7 @@ struct mbg_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::map<Shape, Module> modules;
+    std::map<std::vector<long long>, Shape> tuned;  // measured best shape per model structure
     RunBuffers buf;
     std::unique_ptr<SweepState> sweep;       // last variant sweep
     std::vector<std::unique_ptr<VariantRun>> variant_pool;  // device buffers reused by later sweeps
@@ -413,13 +415,33 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
             }
         }
     }
-    best.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 4)));
+    best.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 2)));
     best.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
     best.sumw = default_sumw(cm.NW, best.wsum_reg);
     if (best_score < 0) {
         if (block_req) throw Error(MBG_ERR_LIMIT, "the requested block does not fit in LDS for this model");
         throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
     }
+    return best;
+}
+
+// The block with the most resident waves for a layout (word-sum placement, tables in LDS or
+// not), optionally only among blocks with fewer than below_waves waves; 0 if nothing fits.
+int layout_block(const mbg_ctx* c, int nw, int wsum_reg, uint32_t blob_bytes, int tables, int* waves_out,
+                 int below_waves = 1 << 30) {
+    const size_t cu_lds = 160 * 1024;
+    int best = 0, best_waves = 0;
+    for (int block = 1024; block >= 64; block -= 64) {
+        const size_t lds = lane_lds_bytes(nw, block, wsum_reg) + (tables ? blob_bytes : 0);
+        if (lds > (size_t)c->lds_per_block) continue;
+        const int waves = std::min(16, (int)(cu_lds / lds) * (block / 64));
+        if (waves >= below_waves) continue;
+        if (waves > best_waves) {
+            best_waves = waves;
+            best = block;
+        }
+    }
+    if (waves_out) *waves_out = best_waves;
     return best;
 }
 
@@ -476,7 +498,24 @@ void unique_fixed_points(const std::vector<uint64_t>& fps, int words, int keep, 
     }
 }
 
-void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
+// Structure of a model that decides which launch shapes are possible and how they perform.
+std::vector<long long> tune_key(const mbg::CompiledModel& cm, const mbg::RateTables& rt, int trace) {
+    return {cm.NW, cm.N, cm.K, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, cm.max_inputs,
+            compact_fits(cm) ? 1 : 0, rates_fit_u32(rt) ? 1 : 0, rates_fit_i31(rt) ? 1 : 0,
+            (long long)cm.deps.size(), (long long)cm.inputs.size(), (long long)cm.lut.size()};
+}
+
+// The shape a run uses without measurement: occupancy model plus tuning knobs.
+Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::RateTables& rt, int block_req, int trace) {
+    const bool rate32 = rates_fit_u32(rt);
+    const bool compact = compact_fits(cm) && env_int("MBG_TUNE_COMPACT", 1) != 0;
+    Shape sh = choose_shape(c, cm, pack_blob(cm, rt, rate32, compact).lay.bytes, block_req, trace, rate32 ? 1 : 0);
+    sh.compact = compact ? 1 : 0;
+    sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
+    return sh;
+}
+
+void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Shape* force = nullptr) {
     const mbg::CompiledModel& cm = m->cm;
     if (cm.K > mbg::kMaxReadouts) throw Error(MBG_ERR_LIMIT, "This executable supports a maximum of 20 non-internal nodes.");
     if (!(cm.time_tick > 0.f) || !(cm.max_time > 0.f)) throw Error(MBG_ERR_ARG, "max_time and time_tick must be positive");
@@ -489,12 +528,17 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     const mbg::RateTables rt = mbg::make_rate_tables(cm, m->mult_up.empty() ? nullptr : m->mult_up.data(),
                                                      m->mult_down.empty() ? nullptr : m->mult_down.data(),
                                                      m->istate.empty() ? nullptr : m->istate.data());
-    const bool rate32 = rates_fit_u32(rt);
-    const bool compact = compact_fits(cm) && env_int("MBG_TUNE_COMPACT", 1) != 0;
-    const Blob blob = pack_blob(cm, rt, rate32, compact);
-    Shape sh = choose_shape(c, cm, blob.lay.bytes, p->block, p->trace, rate32 ? 1 : 0);
-    sh.compact = compact ? 1 : 0;
-    sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
+    Shape sh;
+    if (force) {
+        sh = *force;
+    } else {
+        const auto tu = c->tuned.find(tune_key(cm, rt, p->trace));
+        sh = (!p->block && tu != c->tuned.end()) ? tu->second : default_shape(c, cm, rt, p->block, p->trace);
+    }
+    const Blob blob = pack_blob(cm, rt, sh.rate32 != 0, sh.compact != 0);
+    if ((size_t)lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0) >
+        (size_t)c->lds_per_block)
+        throw Error(MBG_ERR_LIMIT, "launch shape does not fit in LDS");
     const Module& mod = get_module(c, cm, sh);
     const size_t smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0);
     int blocks_per_cu = 0;
@@ -665,6 +709,113 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
     c->have = true;
 }
 
+// Measured launch shape for a model: coordinate descent over the layout (table placement,
+// compact tables, register or LDS rate sums, a lower-occupancy block) and then the step
+// knobs (dependents per round, search width, multiply-add sum updates), each candidate timed
+// on `sample` trajectories (HIP events around the kernel).  Results do not depend on the shape
+// (every draw is keyed by trajectory id and every sum is exact), so only speed is compared.
+// The fastest shape is kept for later runs of models with the same structure.
+void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, mbg_tune_info* out) {
+    const mbg::CompiledModel& cm = m->cm;
+    const mbg::RateTables rt = mbg::make_rate_tables(cm, m->mult_up.empty() ? nullptr : m->mult_up.data(),
+                                                     m->mult_down.empty() ? nullptr : m->mult_down.data(),
+                                                     m->istate.empty() ? nullptr : m->istate.data());
+    const bool r32 = rates_fit_u32(rt), r31 = rates_fit_i31(rt), cfit = compact_fits(cm);
+    const uint64_t n = sample ? sample : std::min<uint64_t>((uint64_t)cm.sample_count, 262144);
+    mbg_run_params rp{};
+    rp.n_trajectories = n;
+    rp.count = n;
+    rp.use_model_seed = 1;
+    rp.trace = trace;
+    int trials = 0;
+    std::set<Shape> tried;
+    auto measure = [&](const Shape& sh) -> double {
+        if (!tried.insert(sh).second) return 1e300;
+        trials++;
+        try {
+            run_impl(c, m, &rp, &sh);
+        } catch (const Error& e) {
+            if (e.code != MBG_ERR_LIMIT) throw;
+            return 1e300;  // does not fit
+        }
+        return c->info.kernel_ms;
+    };
+    const Shape base = default_shape(c, cm, rt, 0, trace);
+    Shape best = base;
+    const double base_ms = measure(base);
+    double best_ms = base_ms;
+    auto consider = [&](const Shape& sh) {
+        const double ms = measure(sh);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = sh;
+        }
+    };
+    // layout: each knob of the default toggled, then one occupancy step down
+    {
+        std::vector<Shape> cands;
+        auto add = [&](int compact, int wreg, int tables, int below) {
+            if (wreg && !(cm.NW <= 16 || (cm.NW <= 32 && cm.NW % 2 == 0))) return;
+            if (compact && !cfit) return;
+            const uint32_t blob = pack_blob(cm, rt, r32, compact != 0).lay.bytes;
+            int waves = 0;
+            const int block = layout_block(c, cm.NW, wreg, blob, tables, &waves, below);
+            if (!block) return;
+            Shape sh = base;
+            sh.compact = compact;
+            sh.wsum_reg = wreg;
+            sh.sumw = wreg ? default_sumw(cm.NW, 1) : 1;
+            sh.wsmad = wreg && r31 ? 1 : 0;
+            sh.tables_lds = tables;
+            sh.block = block;
+            cands.push_back(sh);
+        };
+        add(base.compact, base.wsum_reg, !base.tables_lds, 1 << 30);
+        add(!base.compact, base.wsum_reg, base.tables_lds, 1 << 30);
+        add(base.compact, !base.wsum_reg, base.tables_lds, 1 << 30);
+        {
+            const uint32_t blob = pack_blob(cm, rt, r32, base.compact != 0).lay.bytes;
+            const size_t lds = lane_lds_bytes(cm.NW, base.block, base.wsum_reg) + (base.tables_lds ? blob : 0);
+            const int waves = std::min(16, (int)(160 * 1024 / lds) * (base.block / 64));
+            add(base.compact, base.wsum_reg, base.tables_lds, waves);
+        }
+        for (const Shape& sh : cands) consider(sh);
+    }
+    // step knobs on the best layout
+    {
+        const Shape lay = best;
+        for (int db : {1, 3, 4}) {
+            Shape sh = lay;
+            sh.dep_batch = db;
+            consider(sh);
+        }
+        Shape sh = lay;
+        sh.scan = lay.scan == 8 ? 4 : 8;
+        consider(sh);
+        if (best.wsum_reg && r31) {
+            Shape w = best;
+            w.wsmad = !best.wsmad;
+            consider(w);
+        }
+    }
+    c->tuned[tune_key(cm, rt, trace)] = best;
+    c->have = false;
+    if (out) {
+        *out = mbg_tune_info{};
+        out->block = best.block;
+        out->tables_in_lds = best.tables_lds;
+        out->wsum_reg = best.wsum_reg;
+        out->compact = best.compact;
+        out->dep_batch = best.dep_batch;
+        out->scan = best.scan;
+        out->wsmad = best.wsmad;
+        out->trials = trials;
+        out->sample = n;
+        out->best_ms = best_ms;
+        out->default_ms = base_ms;
+    }
+}
+
 // State-word class of a sweep variant: 1, 2, 4, 8, then multiples of 4 words up to 32, then
 // powers of two.  Lanes of a class hold NWc words, so a tight class keeps more lanes in LDS.
 int word_class(int nw) {
@@ -753,7 +904,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             }
         }
         if (!best_waves) throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
-        C.sh.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 4)));
+        C.sh.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 2)));
         C.sh.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
         const Module& mod = get_module(c, sw->vars[C.members[0]]->m->cm, C.sh);
         C.sim = mod.sim;
@@ -1192,13 +1343,11 @@ mbg_status mbg_compile(mbg_ctx* c, const mbg_model* m, int trace) {
     return guarded(c, [&] {
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         const mbg::CompiledModel& cm = m->cm;
-        const mbg::RateTables rt = mbg::make_rate_tables(cm, nullptr, nullptr, nullptr);
-        const bool r32 = rates_fit_u32(rt);
-        const bool compact = compact_fits(cm) && env_int("MBG_TUNE_COMPACT", 1) != 0;
-        Shape sh = choose_shape(c, cm, pack_blob(cm, rt, r32, compact).lay.bytes, 0, trace, r32 ? 1 : 0);
-        sh.compact = compact ? 1 : 0;
-        sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
-        get_module(c, cm, sh);
+        const mbg::RateTables rt = mbg::make_rate_tables(cm, m->mult_up.empty() ? nullptr : m->mult_up.data(),
+                                                         m->mult_down.empty() ? nullptr : m->mult_down.data(),
+                                                         m->istate.empty() ? nullptr : m->istate.data());
+        const auto tu = c->tuned.find(tune_key(cm, rt, trace));
+        get_module(c, cm, tu != c->tuned.end() ? tu->second : default_shape(c, cm, rt, 0, trace));
     });
 }
 
@@ -1208,6 +1357,15 @@ mbg_status mbg_run(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p) {
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         c->have = false;
         run_impl(c, m, p);
+    });
+}
+
+mbg_status mbg_autotune(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, mbg_tune_info* out) {
+    if (!c || !m) return MBG_ERR_ARG;
+    return guarded(c, [&] {
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        c->have = false;
+        autotune_impl(c, m, sample, trace, out);
     });
 }
 
