@@ -120,6 +120,17 @@ def test_autotuned_shape_gives_identical_results(gpu_ctx):
         assert np.array_equal(tuned.prob_raw, ref.prob_raw) and np.array_equal(tuned.final_states, ref.final_states)
 
 
+@pytest.mark.parametrize("n", [40, 80])
+def test_fixed_points_deduplicated_on_device_bit_exact(gpu_ctx, n):
+    """20,000 trajectories that almost all end at a fixed point (285 and 3,330 distinct
+    states): more than 4,096 records, so the device reduces them to distinct states; the
+    multiset equals the oracle's."""
+    net = synth.random_network(n, 1, 3, 21, rates="loguniform", istate="uniform", readouts=8, special_frac=0.3,
+                               max_time=100.0, time_tick=2.0, sample_count=20000)
+    g, r = _compare(gpu_ctx, net.bnd(), net.cfg(), count=20000, summaries=False)
+    assert g.fixed_point_trajectories > 4096 and len(g.fixed_counts) == len(r.fixed_counts)
+
+
 def test_block_size_and_grid_independent(gpu_ctx):
     net = synth.config_small()
     m = Model(net.bnd(), net.cfg())

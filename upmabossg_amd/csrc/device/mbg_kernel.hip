@@ -884,3 +884,61 @@ extern "C" __global__ __launch_bounds__(256) void mbg_marginals_batch(const mbg_
     if ((int)blockIdx.x >= J.W) return;
     marginals_block(J.prob, J.K, blockIdx.x, out + J.out_off);
 }
+
+// ------------------------------------------------------------------ fixed-point deduplication
+
+// After a run: count the distinct states among the n appended fixed-point records (rows of
+// `stride` words, the first `words` significant) in an open-addressing table of cap_mask + 1
+// slots (at least 2n).  A slot holds the index of the first record of its state; records are
+// immutable here, so a thread that finds a slot taken compares the two records directly and
+// never waits for another thread.  Every record ends in a slot of its own state: an empty slot
+// exists since the table is at most half full.
+extern "C" __global__ __launch_bounds__(256) void mbg_fp_dedup(const u64* states, u64 n, int stride, int words,
+                                                              u32* keys, u64* counts, u64 cap_mask) {
+    const u64 i = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const u64* s = states + i * (u64)stride;
+    u64 h = 0x9E3779B97F4A7C15ull;
+    for (int w = 0; w < words; w++) {
+        h ^= s[w];
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 31;
+    }
+    u64 slot = h & cap_mask;
+    for (;;) {
+        const u32 prev = atomicCAS(keys + slot, 0xFFFFFFFFu, (u32)i);
+        if (prev == 0xFFFFFFFFu) break;
+        const u64* t = states + (u64)prev * (u64)stride;
+        bool eq = true;
+        for (int w = 0; w < words; w++)
+            if (t[w] != s[w]) {
+                eq = false;
+                break;
+            }
+        if (eq) break;
+        slot = (slot + 1) & cap_mask;
+    }
+    atomicAdd(counts + slot, 1ull);
+}
+
+// Occupied slots to (first record index, count) pairs.
+extern "C" __global__ __launch_bounds__(256) void mbg_fp_compact(const u32* keys, const u64* counts, u64 cap, u32* out_idx,
+                                                                u64* out_cnt, u64* out_n) {
+    const u64 slot = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (slot >= cap) return;
+    const u32 k = keys[slot];
+    if (k == 0xFFFFFFFFu) return;
+    const u64 pos = atomicAdd(out_n, 1ull);
+    out_idx[pos] = k;
+    out_cnt[pos] = counts[slot];
+}
+
+// The distinct states: out[j][w] = record idx[j], word w.
+extern "C" __global__ __launch_bounds__(256) void mbg_fp_gather(const u64* states, int stride, int words, const u32* idx,
+                                                               u64 m, u64* out) {
+    const u64 t = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (t >= m * (u64)words) return;
+    const u64 j = t / (u64)words;
+    const int w = (int)(t - j * (u64)words);
+    out[t] = states[(u64)idx[j] * (u64)stride + w];
+}

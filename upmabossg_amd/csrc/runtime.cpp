@@ -219,7 +219,8 @@ Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt, bool rat
 struct Module {
     hipModule_t mod = nullptr;
     hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr,
-                  marginals_batch = nullptr, init_rates_multi = nullptr;
+                  marginals_batch = nullptr, init_rates_multi = nullptr, fp_dedup = nullptr, fp_compact = nullptr,
+                  fp_gather = nullptr;
 };
 
 // Device buffer kept by the context across runs; it only grows.
@@ -267,6 +268,7 @@ struct VariantRun {
     mbg_run_info info{};
     std::vector<uint64_t> fixed_states, fixed_counts;
     std::vector<uint64_t> marg_raw;  // [W][K] readout marginals (exact sums), from the sweep graph
+    const Module* mod = nullptr;     // its class's module (fixed-point deduplication)
 };
 
 // Host mirror of struct mbg_marg_job in device/mbg_kernel.hip.
@@ -313,6 +315,9 @@ struct mbg_ctx {
     std::unique_ptr<SweepState> sweep;       // last variant sweep
     std::vector<std::unique_ptr<VariantRun>> variant_pool;  // device buffers reused by later sweeps
     SweepShared sweep_shared;
+    // fixed-point deduplication on the device
+    DevBuf<uint32_t> fp_keys, fp_idx;
+    DevBuf<uint64_t> fp_counts, fp_cnt, fp_n, fp_out;
     std::vector<hipStream_t> class_streams;  // one per word class of a sweep
     // last run
     bool have = false;
@@ -355,6 +360,9 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
               "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.marginals, m.mod, "mbg_marginals"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.marginals_batch, m.mod, "mbg_marginals_batch"), "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.fp_dedup, m.mod, "mbg_fp_dedup"), "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.fp_compact, m.mod, "mbg_fp_compact"), "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.fp_gather, m.mod, "mbg_fp_gather"), "hipModuleGetFunction");
     if (sh.multi && sh.nw <= 128)
         hip_check(hipModuleGetFunction(&m.init_rates_multi, m.mod, "mbg_init_rates_multi"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_state, m.mod, "mbg_init_state"), "hipModuleGetFunction");
@@ -495,6 +503,72 @@ void unique_fixed_points(const std::vector<uint64_t>& fps, int words, int keep, 
         states.insert(states.end(), row(idx[i]), row(idx[i]) + keep);
         counts.push_back(j - i);
         i = j;
+    }
+}
+
+// Distinct fixed points in the reference order (most significant word first, src/state.h:43-49)
+// from nfp records of `stride` words on the device (the first `keep` significant).  Few records
+// are sorted on the host; many are first reduced to distinct states and counts on the device
+// (mbg_fp_dedup / _compact / _gather), so only those cross PCIe.
+void collect_fixed_points(mbg_ctx* c, const Module& mod, const uint64_t* d_fp, uint64_t nfp, int stride, int keep,
+                          hipStream_t s, std::vector<uint64_t>& states, std::vector<uint64_t>& counts) {
+    states.clear();
+    counts.clear();
+    if (nfp == 0) return;
+    if (nfp <= 4096) {
+        std::vector<uint64_t> fps((size_t)nfp * stride);
+        hip_check(hipMemcpy(fps.data(), d_fp, fps.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        unique_fixed_points(fps, stride, keep, states, counts);
+        return;
+    }
+    uint64_t cap = 1;
+    while (cap < 2 * nfp) cap <<= 1;
+    c->fp_keys.alloc(cap);
+    hip_check(hipMemsetAsync(c->fp_keys.p, 0xFF, cap * 4, s), "hipMemsetAsync");
+    c->fp_counts.zero(cap, s);
+    c->fp_idx.alloc(nfp);
+    c->fp_cnt.alloc(nfp);
+    c->fp_n.zero(1, s);
+    int st = stride, kw = keep;
+    uint64_t n = nfp, mask = cap - 1;
+    const void* fp = d_fp;
+    void* keys = c->fp_keys.p;
+    void* cnts = c->fp_counts.p;
+    void* a1[] = {(void*)&fp, &n, &st, &kw, &keys, &cnts, &mask};
+    hip_check(hipModuleLaunchKernel(mod.fp_dedup, (unsigned)((nfp + 255) / 256), 1, 1, 256, 1, 1, 0, s, a1, nullptr),
+              "launch mbg_fp_dedup");
+    void* oidx = c->fp_idx.p;
+    void* ocnt = c->fp_cnt.p;
+    void* on = c->fp_n.p;
+    void* a2[] = {&keys, &cnts, &cap, &oidx, &ocnt, &on};
+    hip_check(hipModuleLaunchKernel(mod.fp_compact, (unsigned)((cap + 255) / 256), 1, 1, 256, 1, 1, 0, s, a2, nullptr),
+              "launch mbg_fp_compact");
+    uint64_t m = 0;
+    hip_check(hipMemcpyAsync(&m, c->fp_n.p, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s), "fixed-point deduplication");
+    c->fp_out.alloc(m * (uint64_t)keep);
+    void* out = c->fp_out.p;
+    void* a3[] = {(void*)&fp, &st, &kw, &oidx, &m, &out};
+    hip_check(hipModuleLaunchKernel(mod.fp_gather, (unsigned)((m * keep + 255) / 256), 1, 1, 256, 1, 1, 0, s, a3, nullptr),
+              "launch mbg_fp_gather");
+    std::vector<uint64_t> dist((size_t)m * keep), cnt(m);
+    hip_check(hipMemcpyAsync(dist.data(), c->fp_out.p, dist.size() * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(cnt.data(), c->fp_cnt.p, m * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s), "fixed-point gather");
+    std::vector<uint64_t> idx(m);
+    std::iota(idx.begin(), idx.end(), 0);
+    auto row = [&](uint64_t r) { return dist.data() + r * keep; };
+    std::sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) {
+        const uint64_t *x = row(a), *y = row(b);
+        for (int w = keep - 1; w >= 0; w--)
+            if (x[w] != y[w]) return x[w] < y[w];
+        return false;
+    });
+    states.reserve((size_t)m * keep);
+    counts.reserve(m);
+    for (uint64_t r : idx) {
+        states.insert(states.end(), row(r), row(r) + keep);
+        counts.push_back(cnt[r]);
     }
 }
 
@@ -660,31 +734,10 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     hip_check(hipMemcpy(&nfp, d_fpc.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
     hip_check(hipMemcpy(&flips, d_flips.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
     if (nfp > fp_cap) throw Error(MBG_ERR_LIMIT, "fixed-point buffer overflow");
-    std::vector<uint64_t> fps((size_t)nfp * cm.NW);
-    if (nfp) hip_check(hipMemcpy(fps.data(), d_fp.p, fps.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     c->summ.resize((size_t)summ_cap * 4);
     if (summ_cap) hip_check(hipMemcpy(c->summ.data(), d_summ.p, c->summ.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-
     // distinct fixed points in the reference order (src/state.h:43-49)
-    const int NW = cm.NW;
-    std::vector<uint64_t> idx(nfp);
-    std::iota(idx.begin(), idx.end(), 0);
-    auto row = [&](uint64_t r) { return fps.data() + r * NW; };
-    std::sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) {
-        const uint64_t *x = row(a), *y = row(b);
-        for (int w = NW - 1; w >= 0; w--)
-            if (x[w] != y[w]) return x[w] < y[w];
-        return false;
-    });
-    c->fixed_states.clear();
-    c->fixed_counts.clear();
-    for (size_t i = 0; i < idx.size();) {
-        size_t j = i;
-        while (j < idx.size() && std::equal(row(idx[i]), row(idx[i]) + NW, row(idx[j]))) j++;
-        c->fixed_states.insert(c->fixed_states.end(), row(idx[i]), row(idx[i]) + NW);
-        c->fixed_counts.push_back(j - i);
-        i = j;
-    }
+    collect_fixed_points(c, mod, d_fp.p, nfp, cm.NW, cm.NW, s, c->fixed_states, c->fixed_counts);
 
     mbg_run_info& I = c->info;
     I.windows = W;
@@ -907,6 +960,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         C.sh.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 2)));
         C.sh.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
         const Module& mod = get_module(c, sw->vars[C.members[0]]->m->cm, C.sh);
+        for (int v : C.members) sw->vars[v]->mod = &mod;
         C.sim = mod.sim;
         C.init_state = mod.init_state;
         C.init_rates = mod.init_rates;
@@ -1108,9 +1162,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
                               marg.begin() + (ptrdiff_t)(J.out_off + (uint64_t)J.W * (uint64_t)J.K));
         }
         if (nfp > V.D.fp_cap) throw Error(MBG_ERR_LIMIT, "fixed-point buffer overflow");
-        std::vector<uint64_t> fps((size_t)nfp * V.NWc);
-        if (nfp) hip_check(hipMemcpy(fps.data(), V.buf.fp.p, fps.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-        unique_fixed_points(fps, V.NWc, V.m->cm.NW, V.fixed_states, V.fixed_counts);
+        collect_fixed_points(c, *V.mod, V.buf.fp.p, nfp, V.NWc, V.m->cm.NW, main, V.fixed_states, V.fixed_counts);
         V.info.transitions = flips;
         V.info.fixed_point_trajectories = nfp;
         V.info.kernel_ms = ms;
