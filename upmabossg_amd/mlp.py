@@ -150,3 +150,32 @@ def ensemble_predict(members, x):
     if st != 0:
         raise MlpError(lib().mbg_mlp_last_error(members[0]._h).decode())
     return mean, spread, ms.value
+
+
+def _bench(argv=None):
+    """Inference benchmark of the surrogate network: `members` random networks of
+    600 -> 512 x 3 -> 1600 on `points` rows (device time and TFLOP/s of the whole pass)."""
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(prog="python -m upmabossg_amd.mlp")
+    ap.add_argument("--points", type=int, default=10000)
+    ap.add_argument("--members", type=int, default=4)
+    ap.add_argument("--reps", type=int, default=5)
+    a = ap.parse_args(argv)
+    members = [Mlp(600, 512, 3, 1600, max_batch=10240, seed=s) for s in range(a.members)]
+    x = np.random.default_rng(0).uniform(-1, 1, (a.points, 600)).astype(np.float32)
+    dims = [members[0].padded_inputs, 512, 512, 512, 1600]
+    flops = sum(2 * p * q for p, q in zip(dims[:-1], dims[1:])) * a.points * a.members
+    ensemble_predict(members, x)
+    for _ in range(a.reps):
+        t0 = time.perf_counter()
+        _, _, ms = ensemble_predict(members, x)
+        wall = time.perf_counter() - t0
+        print(json.dumps({"points": a.points, "members": a.members, "device_ms": ms, "wall_ms": wall * 1e3,
+                          "tflops": flops / (ms / 1e3) / 1e12}), flush=True)
+
+
+if __name__ == "__main__":
+    _bench()
