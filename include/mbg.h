@@ -63,6 +63,9 @@ mbg_status mbg_model_constant(const mbg_model* m, const char* name, double* out)
  * mutant analysis as future work). */
 mbg_status mbg_model_set_variant(mbg_model* m, const float* mult_up, const float* mult_down,
                                  const float* istate);
+/* The overrides in force: multipliers (1 where none is set) and initial-state probabilities
+ * (the model's own where none is set), N values each; any pointer may be NULL. */
+mbg_status mbg_model_get_variant(const mbg_model* m, float* mult_up, float* mult_down, float* istate);
 
 /* The hipRTC source generated for this model (replaces generator::generate_code,
  * src/generator.cpp:11-55).  Writes at most cap bytes including the NUL; *len gets the full
@@ -180,6 +183,34 @@ mbg_status mbg_get_variant_marginals(const mbg_ctx* c, int v, double* out);
 mbg_status mbg_get_variant_final_states(const mbg_ctx* c, int v, uint64_t* counts);
 mbg_status mbg_get_variant_fixed_points(const mbg_ctx* c, int v, uint64_t* states, uint64_t* counts, size_t cap,
                                         size_t* n);
+
+/* Variants file: the overrides of a mutant / perturbation sweep, one block per variant.
+ *
+ *     variant NAME {                  // NAME: [A-Za-z_][A-Za-z0-9_-]*, unique
+ *         NODE.knockout;              // istate 0, rate_up x 0
+ *         NODE.overexpress;           // istate 1, rate_down x 0
+ *         NODE.rate_up *= NUMBER;     // also rate_down; factors compose
+ *         NODE.istate = NUMBER;       // in [0, 1]
+ *         sample_count = INTEGER;     // default: the model's
+ *         seed_pseudorandom = INTEGER;// default: the model's
+ *     }
+ *
+ * For each block, out[i] gets a new model (a clone of base with the block's overrides applied
+ * on top of base's own; the caller frees it with mbg_model_free), its trajectory count and
+ * seed, and variant_id 0: every variant draws the same Philox numbers per trajectory id as a
+ * plain run of its model (common random numbers), so an empty block reproduces mbg_run on base
+ * bit for bit.  names (optional) gets each NUL-terminated name in 64-byte slots.  *count gets
+ * the number of blocks; with out NULL the file is only checked.  MBG_ERR_PARSE with the line
+ * in err on a syntax error, unknown node or bad value; MBG_ERR_LIMIT if more than cap blocks.
+ * No reference counterpart (README.md:33 lists mutant analysis as future work). */
+mbg_status mbg_variants_parse(const mbg_model* base, const char* text, size_t len, mbg_variant* out, char* names,
+                              int cap, int* count, char* err, size_t errlen);
+mbg_status mbg_variants_parse_file(const mbg_model* base, const char* path, mbg_variant* out, char* names, int cap,
+                                   int* count, char* err, size_t errlen);
+/* The reference's output files and stdout format for variant v of the last sweep (the same
+ * writers as mbg_write_csv / mbg_write_text). */
+mbg_status mbg_write_variant_csv(const mbg_ctx* c, int v, const char* prefix);
+mbg_status mbg_write_variant_text(const mbg_ctx* c, int v, const char* path);
 
 /* Output files of the reference: <prefix>_probtraj.csv (window_average_small_stats::write_csv,
  * src/statistics/window_average_small.cpp:181-250) and <prefix>_fp.csv

@@ -38,15 +38,17 @@ def g(v):
 
 
 class Expected:
-    def __init__(self, bnd, cfg):
+    def __init__(self, bnd, cfg, variant=None, n=None, seed=None):
         o = OracleModel(bnd, cfg)
+        if variant is not None:
+            o.set_variant(*variant)
         self.names = o.names()
         internal = o.internal()
         self.readouts = [i for i in range(len(self.names)) if not internal[i]]
         self.discrete = o.constant("discrete_time") != 0
         self.tick = f32(o.constant("time_tick"))
-        self.n = int(o.constant("sample_count"))
-        self.r = o.run(count=self.n, n_trajectories=self.n, threads=16, record_fixed=True)
+        self.n = n or int(o.constant("sample_count"))
+        self.r = o.run(count=self.n, n_trajectories=self.n, seed=seed, threads=16, record_fixed=True)
 
     def proj(self, s):
         on = [self.names[self.readouts[j]] for j in range(len(self.readouts)) if (s >> j) & 1]
@@ -142,3 +144,42 @@ def test_cli_output_bytes(tmp_path, name):
     assert text.stdout == want.text()
     if name == "metastasis":
         assert len(want.r.fixed_counts) > 0
+
+
+VARIANTS = """
+variant wt { }
+variant ko_p53 { p53.knockout; }
+variant oe_NICD { NICD.overexpress; AKT1.rate_up *= 2.5; }
+variant mix { AKT2.rate_down *= 0.25; CDH1.istate = 1; TGFbeta.rate_up *= 0; sample_count = 700; seed_pseudorandom = 9; }
+"""
+
+
+def test_cli_variants_file_bytes(tmp_path):
+    """-v: one sweep for every variant of the file; each variant's files and stdout block are
+    byte for byte the formatted oracle tables of its model (the overrides applied), and the
+    empty variant's files are the plain run's."""
+    from upmabossg_amd import Model
+
+    bnd, cfg = model_text("metastasis")
+    bp, cp, vp = tmp_path / "m.bnd", tmp_path / "m.cfg", tmp_path / "v.txt"
+    bp.write_text(bnd)
+    cp.write_text(cfg)
+    vp.write_text(VARIANTS)
+    print("case: CLI variants metastasis", flush=True)
+    run = subprocess.run([CLI_PATH, "-o", str(tmp_path / "out"), "-v", str(vp), str(bp), str(cp)],
+                         capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    text = subprocess.run([CLI_PATH, "-v", str(vp), str(bp), str(cp)], capture_output=True, text=True, timeout=120)
+    assert text.returncode == 0, text.stderr
+    plain = subprocess.run([CLI_PATH, "-o", str(tmp_path / "plain"), str(bp), str(cp)], capture_output=True,
+                           text=True, timeout=120)
+    assert plain.returncode == 0, plain.stderr
+    blocks = []
+    for name, model, n, seed in Model(bnd, cfg).variants(VARIANTS):
+        want = Expected(bnd, cfg, variant=model.get_variant(), n=n, seed=seed)
+        assert (tmp_path / f"out_{name}_probtraj.csv").read_text() == want.probtraj_csv(), name
+        assert (tmp_path / f"out_{name}_fp.csv").read_text() == want.fp_csv(), name
+        blocks.append(f"variant {name}\n" + want.text())
+    assert text.stdout == "".join(blocks)
+    assert (tmp_path / "out_wt_probtraj.csv").read_text() == (tmp_path / "plain_probtraj.csv").read_text()
+    assert (tmp_path / "out_wt_fp.csv").read_text() == (tmp_path / "plain_fp.csv").read_text()

@@ -112,6 +112,7 @@ def lib():
         "mbg_model_num_readouts": (C.c_int, [vp]),
         "mbg_model_constant": (C.c_int, [vp, cp, f64p]),
         "mbg_model_set_variant": (C.c_int, [vp, f32p, f32p, f32p]),
+        "mbg_model_get_variant": (C.c_int, [vp, f32p, f32p, f32p]),
         "mbg_model_generate_source": (C.c_int, [vp, C.c_int, cp, sz, C.POINTER(sz)]),
         "mbg_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
         "mbg_destroy": (None, [vp]),
@@ -135,6 +136,11 @@ def lib():
         "mbg_get_variant_fixed_points": (C.c_int, [vp, C.c_int, u64p, u64p, sz, C.POINTER(sz)]),
         "mbg_write_csv": (C.c_int, [vp, vp, cp]),
         "mbg_write_text": (C.c_int, [vp, vp, cp]),
+        "mbg_variants_parse": (C.c_int, [vp, cp, sz, C.POINTER(_Variant), cp, C.c_int, C.POINTER(C.c_int), cp, sz]),
+        "mbg_variants_parse_file": (C.c_int, [vp, cp, C.POINTER(_Variant), cp, C.c_int, C.POINTER(C.c_int), cp,
+                                              sz]),
+        "mbg_write_variant_csv": (C.c_int, [vp, C.c_int, cp]),
+        "mbg_write_variant_text": (C.c_int, [vp, C.c_int, cp]),
     }
     for name, (rt, args) in sig.items():
         f = getattr(L, name)
@@ -230,6 +236,38 @@ class Model:
         st = lib().mbg_model_set_variant(self._h, ptr(mult_up), ptr(mult_down), ptr(istate))
         if st != MBG_OK:
             raise MbgError(st, "invalid variant parameters")
+
+    def get_variant(self):
+        """(mult_up, mult_down, istate) in force, one value per node."""
+        N = self.num_nodes
+        a = [np.zeros(N, np.float32) for _ in range(3)]
+        st = lib().mbg_model_get_variant(self._h, *[x.ctypes.data_as(C.POINTER(C.c_float)) for x in a])
+        if st != MBG_OK:
+            raise MbgError(st, "get_variant failed")
+        return tuple(a)
+
+    def variants(self, text: str):
+        """Parses a variants file (text) against this model: a list of
+        (name, Model, n_trajectories or 0, seed or None), the models carrying the overrides."""
+        L = lib()
+        raw = text.encode()
+        n = C.c_int()
+        err = C.create_string_buffer(4096)
+        st = L.mbg_variants_parse(self._h, raw, len(raw), None, None, 0, C.byref(n), err, 4096)
+        if st != MBG_OK:
+            raise MbgError(st, err.value.decode())
+        arr = (_Variant * max(1, n.value))()
+        names = C.create_string_buffer(64 * max(1, n.value))
+        st = L.mbg_variants_parse(self._h, raw, len(raw), arr, names, n.value, C.byref(n), err, 4096)
+        if st != MBG_OK:
+            raise MbgError(st, err.value.decode())
+        out = []
+        for k in range(n.value):
+            m = Model.__new__(Model)
+            m._h = C.c_void_p(arr[k].model)
+            name = names.raw[64 * k:64 * (k + 1)].split(b"\0", 1)[0].decode()
+            out.append((name, m, arr[k].n_trajectories, None if arr[k].use_model_seed else arr[k].seed))
+        return out
 
     def generate_source(self, trace=False) -> str:
         n = C.c_size_t()
@@ -406,6 +444,12 @@ class Context:
         out = np.zeros(i.windows * i.readouts, np.float64)
         self._check(lib().mbg_get_variant_marginals(self._h, v, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out.reshape(i.windows, i.readouts)
+
+    def write_variant_csv(self, v, prefix: str):
+        self._check(lib().mbg_write_variant_csv(self._h, v, prefix.encode()))
+
+    def write_variant_text(self, v, path: str = "-"):
+        self._check(lib().mbg_write_variant_text(self._h, v, path.encode()))
 
     def write_csv(self, model: Model, prefix: str):
         self._check(lib().mbg_write_csv(self._h, model.handle, prefix.encode()))

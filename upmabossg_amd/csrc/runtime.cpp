@@ -24,6 +24,7 @@
 #include "front.h"
 #include "model.h"
 #include "output.h"
+#include "variants.h"
 
 namespace mbg {
 template <>
@@ -1178,6 +1179,34 @@ const VariantRun& sweep_variant(const mbg_ctx* c, int v) {
     return *c->sweep->vars[v];
 }
 
+// Host copies of one sweep variant's tables, in the form the output writers read.
+struct VariantHost {
+    const VariantRun& V;
+    mbg::PinnedU64 prob;
+    std::vector<uint64_t> th, fin;
+
+    VariantHost(const mbg_ctx* c, const VariantRun& v) : V(v) {
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        const size_t W = (size_t)V.info.windows, S = (size_t)1 << V.info.readouts;
+        prob.resize(W * S);
+        th.resize(W);
+        fin.resize(S);
+        hip_check(hipMemcpy(prob.data(), V.buf.prob.p, W * S * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        hip_check(hipMemcpy(th.data(), V.buf.th.p, W * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        hip_check(hipMemcpy(fin.data(), V.buf.fin.p, S * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    mbg::Results view() const {
+        mbg::Results r;
+        r.info = &V.info;
+        r.prob = &prob;
+        r.th = &th;
+        r.fin = &fin;
+        r.fixed_states = &V.fixed_states;
+        r.fixed_counts = &V.fixed_counts;
+        return r;
+    }
+};
+
 void normalise_marginals(const mbg_run_info& I, const std::vector<uint64_t>& raw, double* out) {
     for (size_t i = 0; i < raw.size(); i++) {
         const double v = (double)raw[i];
@@ -1287,6 +1316,73 @@ mbg_status mbg_model_parse_files(const char* bnd_path, const char* cfg_path, mbg
 
 void mbg_model_free(mbg_model* m) { delete m; }
 
+mbg_status mbg_variants_parse(const mbg_model* base, const char* text, size_t len, mbg_variant* out, char* names,
+                              int cap, int* count, char* err, size_t errlen) {
+    if (!base || !text || !count || cap < 0) return MBG_ERR_ARG;
+    *count = 0;
+    std::vector<mbg::VariantSpec> specs;
+    try {
+        specs = mbg::parse_variants(std::string(text, len), base->cm.names);
+    } catch (const std::exception& e) {
+        copy_err(err, errlen, std::string("variants file, ") + e.what());
+        return MBG_ERR_PARSE;
+    }
+    *count = (int)specs.size();
+    if (!out) return MBG_OK;
+    if ((int)specs.size() > cap) {
+        copy_err(err, errlen, "variants file holds " + std::to_string(specs.size()) + " variants, room for " +
+                                  std::to_string(cap));
+        return MBG_ERR_LIMIT;
+    }
+    const int N = base->cm.N;
+    std::vector<mbg_model*> made;
+    for (const mbg::VariantSpec& s : specs) {
+        auto v = std::make_unique<mbg_model>(*base);
+        std::vector<float> up(N), down(N), ist(N);
+        for (int i = 0; i < N; i++) {
+            up[i] = (base->mult_up.empty() ? 1.f : base->mult_up[i]) * s.mult_up[i];
+            down[i] = (base->mult_down.empty() ? 1.f : base->mult_down[i]) * s.mult_down[i];
+            ist[i] = !std::isnan(s.istate[i]) ? s.istate[i] : base->istate.empty() ? base->cm.istate[i] : base->istate[i];
+        }
+        if (mbg_model_set_variant(v.get(), up.data(), down.data(), ist.data()) != MBG_OK) {
+            for (mbg_model* p : made) delete p;
+            copy_err(err, errlen, "variants file, line " + std::to_string(s.line) + ": variant '" + s.name +
+                                      "' gives rates out of range");
+            return MBG_ERR_PARSE;
+        }
+        made.push_back(v.release());
+    }
+    for (size_t i = 0; i < specs.size(); i++) {
+        const mbg::VariantSpec& s = specs[i];
+        out[i].model = made[i];
+        out[i].n_trajectories = s.has_count ? s.count : 0;
+        out[i].seed = s.seed;
+        out[i].use_model_seed = s.has_seed ? 0 : 1;
+        out[i].variant_id = 0;
+        if (names) {
+            char* dst = names + 64 * i;
+            const size_t k = std::min<size_t>(s.name.size(), 63);
+            std::memcpy(dst, s.name.data(), k);
+            dst[k] = '\0';
+        }
+    }
+    return MBG_OK;
+}
+
+mbg_status mbg_variants_parse_file(const mbg_model* base, const char* path, mbg_variant* out, char* names, int cap,
+                                   int* count, char* err, size_t errlen) {
+    if (!base || !path || !count) return MBG_ERR_ARG;
+    std::string text;
+    try {
+        text = read_file(path);
+    } catch (const std::exception& e) {
+        *count = 0;
+        copy_err(err, errlen, e.what());
+        return MBG_ERR_ARG;
+    }
+    return mbg_variants_parse(base, text.data(), text.size(), out, names, cap, count, err, errlen);
+}
+
 mbg_status mbg_model_clone(const mbg_model* m, mbg_model** out) {
     if (!m || !out) return MBG_ERR_ARG;
     try {
@@ -1343,6 +1439,16 @@ mbg_status mbg_model_set_variant(mbg_model* m, const float* mult_up, const float
         m->mult_up.clear();
         m->mult_down.clear();
         return MBG_ERR_ARG;
+    }
+    return MBG_OK;
+}
+
+mbg_status mbg_model_get_variant(const mbg_model* m, float* mult_up, float* mult_down, float* istate) {
+    if (!m) return MBG_ERR_ARG;
+    for (int i = 0; i < m->cm.N; i++) {
+        if (mult_up) mult_up[i] = m->mult_up.empty() ? 1.f : m->mult_up[i];
+        if (mult_down) mult_down[i] = m->mult_down.empty() ? 1.f : m->mult_down[i];
+        if (istate) istate[i] = m->istate.empty() ? m->cm.istate[i] : m->istate[i];
     }
     return MBG_OK;
 }
@@ -1579,6 +1685,32 @@ mbg_status mbg_get_variant_final_states(const mbg_ctx* c, int v, uint64_t* count
         const VariantRun& V = sweep_variant(c, v);
         const size_t S = (size_t)1 << V.info.readouts;
         hip_check(hipMemcpy(counts, V.buf.fin.p, S * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+mbg_status mbg_write_variant_csv(const mbg_ctx* c, int v, const char* prefix) {
+    if (!c || !prefix) return MBG_ERR_ARG;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        const VariantRun& V = sweep_variant(c, v);
+        const VariantHost h(c, V);
+        if (!mbg::write_probtraj_csv(h.view(), V.m->cm, std::string(prefix) + "_probtraj.csv") ||
+            !mbg::write_fp_csv(h.view(), V.m->cm, std::string(prefix) + "_fp.csv"))
+            throw Error(MBG_ERR_ARG, std::string("cannot write output files with prefix ") + prefix);
+    });
+}
+
+mbg_status mbg_write_variant_text(const mbg_ctx* c, int v, const char* path) {
+    if (!c || !path) return MBG_ERR_ARG;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        const VariantRun& V = sweep_variant(c, v);
+        const VariantHost h(c, V);
+        if (std::string(path) == "-") {
+            mbg::write_text(h.view(), V.m->cm, std::cout);
+        } else {
+            std::ofstream f(path);
+            if (!f) throw Error(MBG_ERR_ARG, std::string("cannot open ") + path);
+            mbg::write_text(h.view(), V.m->cm, f);
+        }
     });
 }
 
