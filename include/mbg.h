@@ -138,6 +138,16 @@ mbg_status mbg_get_probtraj_raw(const mbg_ctx* c, uint64_t* prob, uint64_t* th);
 mbg_status mbg_get_probtraj(const mbg_ctx* c, double* prob, double* th);
 /* Final projected states, counts[2^K] (final_states_stats, src/statistics/final_states.cpp:45-66). */
 mbg_status mbg_get_final_states(const mbg_ctx* c, uint64_t* counts);
+/* The non-zero window cells, window-major with projected states ascending: window index,
+ * state (bit j = readout j) and exact fixed-point value (count when discrete), for any number
+ * of readouts.  Runs with more than 20 readouts (the reference's limit, src/main.cpp:142-146)
+ * keep their cells in a device hash table and have only this form: the dense getters above
+ * return MBG_ERR_LIMIT for them.  *n gets the number of cells; at most cap are written (any
+ * array may be NULL). */
+mbg_status mbg_get_probtraj_cells(const mbg_ctx* c, uint32_t* windows, uint64_t* states, uint64_t* cells, size_t cap,
+                                  size_t* n);
+/* The non-zero final-state counts, states ascending (any number of readouts). */
+mbg_status mbg_get_final_cells(const mbg_ctx* c, uint64_t* states, uint64_t* counts, size_t cap, size_t* n);
 /* Distinct fixed points in the reference order (most significant word first,
  * src/state.h:43-49; fixed_states_stats, src/statistics/fixed_states.cu:84-187): states
  * [cap][state_words] and counts [cap]; *n gets the number of distinct fixed points. */

@@ -803,7 +803,7 @@ static void build(orc_model& m) {
         c.refs.assign(refs.begin(), refs.end());
         m.max_stack = std::max(m.max_stack, c.logic.size() + c.rate_up.size() + c.rate_down.size() + 4);
     }
-    if (m.readouts.size() > 20) throw Error("This executable supports a maximum of 20 non-internal nodes.");
+    if (m.readouts.size() > 63) throw Error("This executable supports a maximum of 63 non-internal nodes.");
     compute_exponent(m);
 }
 
@@ -814,6 +814,25 @@ struct Tables {
     int W = 0, K = 0, FP = 0, FT = 0;
     uint64_t S = 0;
     std::vector<uint64_t> prob, th, fin;
+    // more than 20 readouts: per-thread maps of the non-zero cells ((window, state) -> sum) and
+    // final states, merged after the run into sorted lists (window-major, states ascending)
+    bool sparse = false;
+    std::vector<std::map<std::pair<uint32_t, uint64_t>, uint64_t>> cell_maps;
+    std::vector<std::map<uint64_t, uint64_t>> fin_maps;
+    std::vector<uint32_t> cw;
+    std::vector<uint64_t> cs, cv, fs, fv;
+
+    void cell(int tix, int w, uint64_t s, uint64_t v) {
+        if (!sparse) {
+            __atomic_fetch_add(&prob[(uint64_t)w * S + s], v, __ATOMIC_RELAXED);
+        } else if (v) {
+            cell_maps[tix][{(uint32_t)w, s}] += v;
+        }
+    }
+    void final_state(int tix, uint64_t s) {
+        if (!sparse) __atomic_fetch_add(&fin[s], 1, __ATOMIC_RELAXED);
+        else fin_maps[tix][s] += 1;
+    }
 };
 
 static inline void atomic_add(uint64_t* p, uint64_t v) { __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
@@ -860,7 +879,7 @@ static double readout_entropy(const orc_model& m, const uint64_t* r) {
     return th < 0.0 ? 0.0 : th;
 }
 
-static void simulate_range(const orc_model& m, const orc_params& p, uint64_t b, uint64_t e, Tables& T,
+static void simulate_range(const orc_model& m, const orc_params& p, uint64_t b, uint64_t e, Tables& T, int tix,
                            std::vector<std::vector<uint64_t>>* fixed, orc_summary* sums, uint64_t* flips_out) {
     const int N = m.N, NW = m.NW;
     const bool discrete = m.d.constants.at("discrete_time") != 0.f;
@@ -922,7 +941,7 @@ static void simulate_range(const orc_model& m, const orc_params& p, uint64_t b, 
             if (discrete) {
                 int w = (int)floor(t * inv_tick + 0.5);
                 if (w > T.W - 1) w = T.W - 1;
-                atomic_add(&T.prob[(uint64_t)w * T.S + s], 1);
+                T.cell(tix, w, s, 1);
                 atomic_add(&T.th[w], (uint64_t)nearbyint(ldexp(th, T.FT)));
             } else {
                 int w = (int)floor(t * inv_tick);
@@ -932,7 +951,7 @@ static void simulate_range(const orc_model& m, const orc_params& p, uint64_t b, 
                     double en = t1 < we ? t1 : we;
                     double sl = en - bgn;
                     int ww = w > T.W - 1 ? T.W - 1 : w;
-                    atomic_add(&T.prob[(uint64_t)ww * T.S + s], (uint64_t)nearbyint(ldexp(sl, T.FP)));
+                    T.cell(tix, ww, s, (uint64_t)nearbyint(ldexp(sl, T.FP)));
                     atomic_add(&T.th[ww], (uint64_t)nearbyint(ldexp(th * sl, T.FT)));
                     w++;
                     bgn = en;
@@ -966,7 +985,7 @@ static void simulate_range(const orc_model& m, const orc_params& p, uint64_t b, 
         }
         uint64_t s = 0;
         for (size_t j = 0; j < m.readouts.size(); j++) s |= ((x[m.readouts[j] >> 6] >> (m.readouts[j] & 63)) & 1ull) << j;
-        atomic_add(&T.fin[s], 1);
+        T.final_state(tix, s);
         if (fixed_pt && fixed) fixed->push_back(x);
         if (sums) {
             orc_summary& S = sums[id - b];
@@ -1065,7 +1084,8 @@ orc_result* orc_run(orc_model* m, const orc_params* p, char* err, size_t errlen)
         const bool discrete = m->d.constants.at("discrete_time") != 0.f;
         T.W = (int)std::ceil(maxt_f / tick_f);  // window_average_small.cpp:26 (float division)
         T.K = (int)m->readouts.size();
-        T.S = 1ull << T.K;
+        T.sparse = T.K > 20;
+        T.S = T.sparse ? 0 : 1ull << T.K;
         uint64_t n = p->n_trajectories ? p->n_trajectories : 1;
         if (discrete) {
             T.FP = 0;
@@ -1083,6 +1103,10 @@ orc_result* orc_run(orc_model* m, const orc_params* p, char* err, size_t errlen)
         uint64_t cnt = p->count;
         if ((uint64_t)nt > cnt) nt = (int)std::max<uint64_t>(1, cnt);
         std::vector<std::vector<std::vector<uint64_t>>> fx(nt);
+        if (T.sparse) {
+            T.cell_maps.resize(nt);
+            T.fin_maps.resize(nt);
+        }
         std::vector<uint64_t> fl(nt, 0);
         if (p->record_summaries) res->summaries.resize(cnt);
         auto t0 = std::chrono::steady_clock::now();
@@ -1090,13 +1114,44 @@ orc_result* orc_run(orc_model* m, const orc_params* p, char* err, size_t errlen)
         for (int w = 0; w < nt; w++) {
             uint64_t b = p->first_id + cnt * w / nt, e = p->first_id + cnt * (w + 1) / nt;
             orc_summary* sp = p->record_summaries ? res->summaries.data() + (b - p->first_id) : nullptr;
-            th.emplace_back(simulate_range, std::cref(*m), std::cref(*p), b, e, std::ref(T),
+            th.emplace_back(simulate_range, std::cref(*m), std::cref(*p), b, e, std::ref(T), w,
                             p->record_fixed ? &fx[w] : nullptr, sp, &fl[w]);
         }
         for (auto& t : th) t.join();
         auto t1 = std::chrono::steady_clock::now();
         res->seconds = std::chrono::duration<double>(t1 - t0).count();
         for (auto f : fl) res->flips += f;
+        if (T.sparse) {
+            std::map<std::pair<uint32_t, uint64_t>, uint64_t> cells;
+            std::map<uint64_t, uint64_t> fins;
+            for (auto& mp : T.cell_maps)
+                for (auto& kv : mp) cells[kv.first] += kv.second;
+            for (auto& mp : T.fin_maps)
+                for (auto& kv : mp) fins[kv.first] += kv.second;
+            T.cell_maps.clear();
+            T.fin_maps.clear();
+            for (auto& kv : cells) {
+                T.cw.push_back(kv.first.first);
+                T.cs.push_back(kv.first.second);
+                T.cv.push_back(kv.second);
+            }
+            for (auto& kv : fins) {
+                T.fs.push_back(kv.first);
+                T.fv.push_back(kv.second);
+            }
+        } else {
+            for (uint64_t i = 0; i < T.prob.size(); i++)
+                if (T.prob[i]) {
+                    T.cw.push_back((uint32_t)(i / T.S));
+                    T.cs.push_back(i % T.S);
+                    T.cv.push_back(T.prob[i]);
+                }
+            for (uint64_t i = 0; i < T.fin.size(); i++)
+                if (T.fin[i]) {
+                    T.fs.push_back(i);
+                    T.fv.push_back(T.fin[i]);
+                }
+        }
         // fixed points ordered as static_state_t::operator< (src/state.h:43-49)
         std::vector<std::vector<uint64_t>> all;
         for (auto& v : fx)
@@ -1129,6 +1184,13 @@ int orc_th_exponent(const orc_result* r) { return r->t.FT; }
 const uint64_t* orc_prob_table(const orc_result* r) { return r->t.prob.data(); }
 const uint64_t* orc_th_table(const orc_result* r) { return r->t.th.data(); }
 const uint64_t* orc_final_table(const orc_result* r) { return r->t.fin.data(); }
+uint64_t orc_cell_count(const orc_result* r) { return r->t.cv.size(); }
+const uint32_t* orc_cell_windows(const orc_result* r) { return r->t.cw.data(); }
+const uint64_t* orc_cell_states(const orc_result* r) { return r->t.cs.data(); }
+const uint64_t* orc_cell_values(const orc_result* r) { return r->t.cv.data(); }
+uint64_t orc_final_cell_count(const orc_result* r) { return r->t.fv.size(); }
+const uint64_t* orc_final_cell_states(const orc_result* r) { return r->t.fs.data(); }
+const uint64_t* orc_final_cell_counts(const orc_result* r) { return r->t.fv.data(); }
 uint64_t orc_total_flips(const orc_result* r) { return r->flips; }
 double orc_elapsed_seconds(const orc_result* r) { return r->seconds; }
 uint64_t orc_fixed_count(const orc_result* r) { return r->fixed_counts.size(); }

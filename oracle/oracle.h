@@ -89,6 +89,15 @@ int orc_th_exponent(const orc_result* r);    /* F_TH */
 const uint64_t* orc_prob_table(const orc_result* r);
 const uint64_t* orc_th_table(const orc_result* r);
 const uint64_t* orc_final_table(const orc_result* r);
+/* The non-zero cells, window-major with states ascending (every run; the only form of the
+ * window table and final states above 20 readouts, where prob and final tables are empty). */
+uint64_t orc_cell_count(const orc_result* r);
+const uint32_t* orc_cell_windows(const orc_result* r);
+const uint64_t* orc_cell_states(const orc_result* r);
+const uint64_t* orc_cell_values(const orc_result* r);
+uint64_t orc_final_cell_count(const orc_result* r);
+const uint64_t* orc_final_cell_states(const orc_result* r);
+const uint64_t* orc_final_cell_counts(const orc_result* r);
 uint64_t orc_total_flips(const orc_result* r);
 double orc_elapsed_seconds(const orc_result* r);
 /* Fixed points: sorted ascending by the reference order (most significant word first). */

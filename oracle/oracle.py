@@ -81,7 +81,12 @@ def lib():
     for n in ["orc_prob_table", "orc_th_table", "orc_final_table", "orc_fixed_states", "orc_fixed_counts"]:
         getattr(L, n).restype = u64p
         getattr(L, n).argtypes = [vp]
-    for n in ["orc_total_flips", "orc_fixed_count"]:
+    for n in ["orc_cell_states", "orc_cell_values", "orc_final_cell_states", "orc_final_cell_counts"]:
+        getattr(L, n).restype = u64p
+        getattr(L, n).argtypes = [vp]
+    L.orc_cell_windows.restype = C.POINTER(C.c_uint32)
+    L.orc_cell_windows.argtypes = [vp]
+    for n in ["orc_total_flips", "orc_fixed_count", "orc_cell_count", "orc_final_cell_count"]:
         getattr(L, n).restype = C.c_uint64
         getattr(L, n).argtypes = [vp]
     L.orc_elapsed_seconds.restype = C.c_double
@@ -114,14 +119,16 @@ class OracleResult:
     readouts: int
     prob_exponent: int
     th_exponent: int
-    prob: np.ndarray  # (W, S) uint64 fixed point
+    prob: np.ndarray | None  # (W, S) uint64 fixed point; None above 20 readouts
     th: np.ndarray  # (W,) uint64 fixed point
-    final: np.ndarray  # (S,) uint64 counts
+    final: np.ndarray | None  # (S,) uint64 counts; None above 20 readouts
     flips: int
     seconds: float
     fixed_states: np.ndarray  # (F, words) uint64, reference order
     fixed_counts: np.ndarray  # (F,) uint64
     summaries: np.ndarray | None
+    cells: tuple = None  # non-zero cells (window uint32, state uint64, value uint64), window-major
+    final_cells: tuple = None  # non-zero final states (state, count), ascending
 
 
 class OracleModel:
@@ -218,10 +225,22 @@ class OracleModel:
         try:
             W = L.orc_windows(r)
             K = L.orc_readouts(r)
-            S = 1 << K
-            prob = np.ctypeslib.as_array(L.orc_prob_table(r), shape=(W * S,)).copy().reshape(W, S)
             th = np.ctypeslib.as_array(L.orc_th_table(r), shape=(W,)).copy()
-            fin = np.ctypeslib.as_array(L.orc_final_table(r), shape=(S,)).copy()
+            if K > 20:
+                prob = fin = None
+            else:
+                S = 1 << K
+                prob = np.ctypeslib.as_array(L.orc_prob_table(r), shape=(W * S,)).copy().reshape(W, S)
+                fin = np.ctypeslib.as_array(L.orc_final_table(r), shape=(S,)).copy()
+
+            def arr(ptr, n, dt):
+                return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n else np.zeros(0, dt)
+
+            nc, nf = L.orc_cell_count(r), L.orc_final_cell_count(r)
+            cells = (arr(L.orc_cell_windows(r), nc, np.uint32), arr(L.orc_cell_states(r), nc, np.uint64),
+                     arr(L.orc_cell_values(r), nc, np.uint64))
+            final_cells = (arr(L.orc_final_cell_states(r), nf, np.uint64),
+                           arr(L.orc_final_cell_counts(r), nf, np.uint64))
             F = L.orc_fixed_count(r)
             NW = L.orc_state_words(r)
             if F:
@@ -236,6 +255,6 @@ class OracleModel:
                 buf = (C.c_char * (count * SUMMARY_DTYPE.itemsize)).from_address(sp)
                 summ = np.frombuffer(bytes(buf), dtype=SUMMARY_DTYPE).copy()
             return OracleResult(W, K, L.orc_prob_exponent(r), L.orc_th_exponent(r), prob, th, fin,
-                                L.orc_total_flips(r), L.orc_elapsed_seconds(r), fs, fc, summ)
+                                L.orc_total_flips(r), L.orc_elapsed_seconds(r), fs, fc, summ, cells, final_cells)
         finally:
             L.orc_result_free(r)

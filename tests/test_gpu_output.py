@@ -58,8 +58,8 @@ class Expected:
         on = [self.names[i] for i in range(len(self.names)) if (int(words[i >> 6]) >> (i & 63)) & 1]
         return " -- ".join(on) if on else "<nil>"
 
-    def prob(self, w, s):
-        v = float(self.r.prob[w, s])
+    def prob(self, raw):
+        v = float(raw)
         if self.discrete:
             return f32(v / self.n)
         return f32(math.ldexp(v, -self.r.prob_exponent) / (self.n * float(self.tick)))
@@ -71,9 +71,13 @@ class Expected:
         return f32(v)
 
     def row(self, w):
+        # the oracle's non-zero cells (window-major, states ascending): any number of readouts
+        cw, cs, cv = self.r.cells
+        lo, hi = np.searchsorted(cw, w, "left"), np.searchsorted(cw, w, "right")
         probs, ent = [], f32(0)
-        for s in range(self.r.prob.shape[1]):
-            p = self.prob(w, s)
+        for s, raw in zip(cs[lo:hi], cv[lo:hi]):
+            s = int(s)
+            p = self.prob(raw)
             if p == 0:
                 continue
             ent = f32(ent + f32(f32(-_LIBM.log2f(p)) * p))
@@ -102,9 +106,8 @@ class Expected:
     def text(self):
         n = f32(self.n)
         out = "final points:\n"
-        for s, c in enumerate(self.r.final):
-            if c:
-                out += f"{g(f32(c) / n)} {self.proj(s)}\n"
+        for s, c in zip(*self.r.final_cells):
+            out += f"{g(f32(c) / n)} {self.proj(int(s))}\n"
         out += "fixed points:\n"
         for i, c in enumerate(self.r.fixed_counts):
             out += f"{g(f32(c) / n)} {self.full(self.r.fixed_states[i])}\n"
@@ -120,10 +123,20 @@ class Expected:
 CASES = {
     "metastasis": lambda: model_text("metastasis"),  # continuous, 1,000 trajectories, fixed points
     "cellcycle": lambda: (model_text("cellcycle")[0], model_text("cellcycle")[1] + "\nsample_count = 20000;\n"),
+    "readouts24": lambda: _readouts24(),
     "frozen": lambda: ("node A { logic = A; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n"
                        "node B { logic = !A; rate_up = @logic ? 2 : 0; rate_down = @logic ? 0 : 2; }\n",
                        "A.istate = 0.5; B.istate = 0.5; max_time = 5; time_tick = 1; sample_count = 1000;"),
 }
+
+
+def _readouts24():
+    # more than 20 non-internal nodes: sparse window cells (beyond the reference's limit)
+    from upmabossg_amd import synth
+
+    net = synth.random_network(48, 2, 3, seed=11, rates="loguniform", istate="uniform", readouts=24, max_time=10.0,
+                               time_tick=0.5, sample_count=3000)
+    return net.bnd(), net.cfg()
 
 
 @pytest.mark.parametrize("name", sorted(CASES))

@@ -38,9 +38,15 @@ def _compare(ctx, bnd, cfg, count, n=None, first_id=0, seed=None, threads=16, su
         assert np.array_equal(g.summaries[:, 3], s["status"].astype(np.uint64)), "statuses differ"
     assert g.transitions == r.flips
     assert g.prob_exponent == r.prob_exponent and g.th_exponent == r.th_exponent
-    assert np.array_equal(g.prob_raw, r.prob), "window tables differ"
+    if r.readouts > 20:  # sparse cells
+        for a, b in zip(g.cells, r.cells):
+            assert np.array_equal(a, b), "window cells differ"
+        for a, b in zip(g.final_cells, r.final_cells):
+            assert np.array_equal(a, b), "final states differ"
+    else:
+        assert np.array_equal(g.prob_raw, r.prob), "window tables differ"
+        assert np.array_equal(g.final_states, r.final), "final states differ"
     assert np.array_equal(g.th_raw, r.th), "window entropies differ"
-    assert np.array_equal(g.final_states, r.final), "final states differ"
     assert np.array_equal(g.fixed_states, r.fixed_states), "fixed points differ"
     assert np.array_equal(g.fixed_counts, r.fixed_counts), "fixed-point counts differ"
     return g, r
@@ -243,3 +249,28 @@ def test_large_full_size_properties(gpu_ctx):
     po = (r.prob.astype(np.float64) / 2.0 ** r.prob_exponent / (4096 * 0.5)) @ bits
     sig = np.sqrt(np.maximum(pg * (1 - pg), 1e-4) / 4096)
     assert np.all(np.abs(pg - po) <= 4.5 * sig)
+
+
+@pytest.mark.parametrize("n,readouts,discrete,count", [(48, 24, False, 4000), (300, 40, False, 2000),
+                                                      (64, 58, True, 3000), (2000, 32, False, 256)])
+def test_more_than_20_readouts_bit_exact(gpu_ctx, n, readouts, discrete, count):
+    """Sparse window cells (device hash table keyed by window and projected state): the cells,
+    final states, entropies and fixed points equal the oracle's, for 24 to 58 readouts (58 + 5
+    bits of window index fill the 63-bit key); the first two cases hold more cells than the
+    table they start from (2^16 slots), so the run is repeated with a larger one."""
+    net = synth.random_network(n, 2, 3, seed=n + readouts, rates="loguniform", istate="uniform", readouts=readouts,
+                               max_time=10.0, time_tick=0.5, sample_count=count)
+    net.discrete = discrete
+    g, r = _compare(gpu_ctx, net.bnd(), net.cfg(), count=count)
+    assert g.prob_raw is None and g.readouts == readouts
+    w, st, v = g.cells
+    if n <= 300 and not discrete:
+        assert len(v) > 2 ** 16  # more cells than the starting table: the growth path ran
+    # the readout marginals of the sparse cells
+    K = readouts
+    raw = np.zeros((g.windows, K))
+    for j in range(K):
+        on = ((st >> np.uint64(j)) & np.uint64(1)).astype(bool)
+        np.add.at(raw[:, j], w[on].astype(np.int64), v[on].astype(np.float64))
+    want = raw / count if discrete else raw / (count * float(np.float32(net.time_tick)) * 2.0 ** g.prob_exponent)
+    assert np.allclose(gpu_ctx.marginals(), want, rtol=1e-9, atol=0)

@@ -87,11 +87,15 @@ def test_parse_errors(bnd, cfg, status, msg):
     assert msg.lower() in str(e.value).lower()
 
 
-def test_more_than_20_readouts_is_a_limit_error():
-    net = synth.random_network(21, 1, 2, seed=1, readouts=21)
+def test_more_than_63_readouts_is_a_limit_error():
+    # up to 20 readouts: dense window tables (the reference's limit, src/main.cpp:142-146);
+    # 21 to 63: sparse tables; above: a limit error
+    net = synth.random_network(64, 1, 2, seed=1, readouts=64)
     with pytest.raises(MbgError) as e:
         Model(net.bnd(), net.cfg())
-    assert e.value.status == 2
+    assert e.value.status == 2 and "63 non-internal" in str(e.value)
+    net = synth.random_network(63, 1, 2, seed=1, readouts=63)
+    assert Model(net.bnd(), net.cfg()).num_readouts == 63
 
 
 def test_grammar_details():

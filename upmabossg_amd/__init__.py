@@ -124,6 +124,8 @@ def lib():
         "mbg_get_probtraj_raw": (C.c_int, [vp, u64p, u64p]),
         "mbg_get_probtraj": (C.c_int, [vp, f64p, f64p]),
         "mbg_get_final_states": (C.c_int, [vp, u64p]),
+        "mbg_get_probtraj_cells": (C.c_int, [vp, C.POINTER(C.c_uint32), u64p, u64p, sz, C.POINTER(sz)]),
+        "mbg_get_final_cells": (C.c_int, [vp, u64p, u64p, sz, C.POINTER(sz)]),
         "mbg_get_fixed_points": (C.c_int, [vp, u64p, u64p, sz, C.POINTER(sz)]),
         "mbg_get_summaries": (C.c_int, [vp, u64p, sz]),
         "mbg_get_marginals": (C.c_int, [vp, f64p]),
@@ -292,14 +294,18 @@ class RunResult:
     kernel_ms: float
     time_tick: float
     max_time: float
-    prob_raw: np.ndarray       # (W, 2^K) uint64 fixed point (counts when discrete)
-    th_raw: np.ndarray         # (W,) uint64 fixed point
-    probtraj: np.ndarray       # (W, 2^K) float64 normalised
-    th: np.ndarray             # (W,) float64
-    final_states: np.ndarray   # (2^K,) uint64 counts
-    fixed_states: np.ndarray   # (F, words) uint64, reference order
-    fixed_counts: np.ndarray   # (F,) uint64
-    summaries: np.ndarray | None  # (count, 4) uint64 when traced
+    prob_raw: np.ndarray | None      # (W, 2^K) uint64 fixed point (counts when discrete); None above 20 readouts
+    th_raw: np.ndarray               # (W,) uint64 fixed point
+    probtraj: np.ndarray | None      # (W, 2^K) float64 normalised; None above 20 readouts
+    th: np.ndarray                   # (W,) float64
+    final_states: np.ndarray | None  # (2^K,) uint64 counts; None above 20 readouts
+    fixed_states: np.ndarray         # (F, words) uint64, reference order
+    fixed_counts: np.ndarray         # (F,) uint64
+    summaries: np.ndarray | None     # (count, 4) uint64 when traced
+    # above 20 readouts: the non-zero cells (window uint32, state uint64, raw uint64), window-major
+    # with states ascending, and the final states (state, count), states ascending
+    cells: tuple | None = None
+    final_cells: tuple | None = None
 
 
 class Context:
@@ -354,19 +360,45 @@ class Context:
         self._check(lib().mbg_get_run_info(self._h, C.byref(i)))
         return i
 
+    def cells(self):
+        """The last run's non-zero window cells (window, state, raw value), window-major with
+        states ascending, and its final states (state, count); any number of readouts."""
+        L = lib()
+        n = C.c_size_t()
+        self._check(L.mbg_get_probtraj_cells(self._h, None, None, None, 0, C.byref(n)))
+        w = np.zeros(n.value, np.uint32)
+        st = np.zeros(n.value, np.uint64)
+        v = np.zeros(n.value, np.uint64)
+        self._check(L.mbg_get_probtraj_cells(self._h, w.ctypes.data_as(C.POINTER(C.c_uint32)), _u64p(st), _u64p(v),
+                                             n.value, C.byref(n)))
+        self._check(L.mbg_get_final_cells(self._h, None, None, 0, C.byref(n)))
+        fs = np.zeros(n.value, np.uint64)
+        fc = np.zeros(n.value, np.uint64)
+        self._check(L.mbg_get_final_cells(self._h, _u64p(fs), _u64p(fc), n.value, C.byref(n)))
+        return (w, st, v), (fs, fc)
+
     def result(self, with_summaries=False) -> RunResult:
         L = lib()
         i = self.info()
-        W, S = i.windows, 1 << i.readouts
-        prob_raw = np.zeros(W * S, np.uint64)
+        W = i.windows
         th_raw = np.zeros(W, np.uint64)
-        self._check(L.mbg_get_probtraj_raw(self._h, _u64p(prob_raw), _u64p(th_raw)))
-        prob = np.zeros(W * S, np.float64)
         th = np.zeros(W, np.float64)
-        self._check(L.mbg_get_probtraj(self._h, prob.ctypes.data_as(C.POINTER(C.c_double)),
-                                       th.ctypes.data_as(C.POINTER(C.c_double))))
-        fin = np.zeros(S, np.uint64)
-        self._check(L.mbg_get_final_states(self._h, _u64p(fin)))
+        cells = final_cells = None
+        if i.readouts > 20:
+            self._check(L.mbg_get_probtraj_raw(self._h, None, _u64p(th_raw)))
+            self._check(L.mbg_get_probtraj(self._h, None, th.ctypes.data_as(C.POINTER(C.c_double))))
+            cells, final_cells = self.cells()
+            prob_raw = prob = fin = None
+        else:
+            S = 1 << i.readouts
+            prob_raw = np.zeros(W * S, np.uint64)
+            self._check(L.mbg_get_probtraj_raw(self._h, _u64p(prob_raw), _u64p(th_raw)))
+            prob = np.zeros(W * S, np.float64)
+            self._check(L.mbg_get_probtraj(self._h, prob.ctypes.data_as(C.POINTER(C.c_double)),
+                                           th.ctypes.data_as(C.POINTER(C.c_double))))
+            fin = np.zeros(S, np.uint64)
+            self._check(L.mbg_get_final_states(self._h, _u64p(fin)))
+            prob_raw, prob = prob_raw.reshape(W, S), prob.reshape(W, S)
         n = C.c_size_t()
         self._check(L.mbg_get_fixed_points(self._h, None, None, 0, C.byref(n)))
         fs = np.zeros((n.value, i.state_words), np.uint64)
@@ -379,8 +411,7 @@ class Context:
             self._check(L.mbg_get_summaries(self._h, _u64p(summ), i.simulated))
         return RunResult(W, i.readouts, i.prob_exponent, i.th_exponent, i.rate_exponent, bool(i.discrete),
                          i.n_trajectories, i.simulated, i.transitions, i.fixed_point_trajectories, i.kernel_ms,
-                         i.time_tick, i.max_time, prob_raw.reshape(W, S), th_raw, prob.reshape(W, S), th, fin,
-                         fs, fc, summ)
+                         i.time_tick, i.max_time, prob_raw, th_raw, prob, th, fin, fs, fc, summ, cells, final_cells)
 
     def probtraj_raw(self, out=None) -> np.ndarray:
         """The last run's window table (W * 2^K exact cells), copied to host memory."""

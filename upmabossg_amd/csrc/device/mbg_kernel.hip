@@ -16,6 +16,7 @@
 //   MBG_SUMW        64-bit words per rate-sum group (register sums only; divides MBG_NW)
 //   MBG_COMPACT     1: 8-byte node records and 16-bit dependents (small models, no generic rates)
 //   MBG_WSMAD       1: every rate is below 2^31 (register sum updates by multiply-add)
+//   MBG_SPARSE      1: more than 20 readouts, window cells and final states in a hash table
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -66,6 +67,15 @@ struct mbg_node {
 
 #ifndef MBG_COMPACT
 #define MBG_COMPACT 0
+#endif
+#ifndef MBG_SPARSE
+#define MBG_SPARSE 0
+#endif
+// Projected (readout) state: up to 20 bits in a dense table, up to 63 with MBG_SPARSE.
+#if MBG_SPARSE
+typedef unsigned long long sidx;
+#else
+typedef unsigned int sidx;
 #endif
 #ifndef MBG_WSMAD
 #define MBG_WSMAD 0
@@ -141,6 +151,12 @@ struct mbg_desc {
     int N, K, W, FP, FT, sigma;
     int compat;               // 1: no flip after every 99th step (the reference's 100-record launches)
     int pad_;
+    // MBG_SPARSE (more than 20 readouts): window cells and final states in one open-addressing
+    // table instead of prob / fin; key = window << K | projected state (window W: final states)
+    u64* cells;               // [cell_mask + 1] keys, ~0 = empty
+    u64* cell_vals;           // [cell_mask + 1] fixed-point sums / counts
+    u64* cell_miss;           // inserts that found no free slot within the probe limit
+    u64 cell_mask;
 };
 
 #if MBG_COMPACT
@@ -433,8 +449,43 @@ struct Lane {
     }
 };
 
-__device__ __forceinline__ void flush_cell(const mbg_desc& D, int w, u32 s, u64 v) {
+#if MBG_SPARSE
+#define MBG_EMPTY_CELL (~0ull)
+#define MBG_CELL_PROBES 64
+// Adds v to the cell of `key` in the open-addressing table: linear probing from a 64-bit mix
+// of the key (the MurmurHash3 finalizer), at most MBG_CELL_PROBES slots.  A slot's key is
+// written once (compare-and-swap on empty) and never moves, so no lane waits on another; a
+// key that finds no slot is counted in cell_miss and the host repeats the run with a larger
+// table (the run is deterministic, so the repeat gives the same cells).
+__device__ __forceinline__ void sparse_add(const mbg_desc& D, u64 key, u64 v) {
+    u64 h = key;
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ull;
+    h ^= h >> 33;
+    for (int probe = 0; probe < MBG_CELL_PROBES; probe++) {
+        const u64 slot = (h + (u64)probe) & D.cell_mask;
+        u64 cur = __hip_atomic_load(D.cells + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == MBG_EMPTY_CELL) {
+            cur = atomicCAS(D.cells + slot, MBG_EMPTY_CELL, key);
+            if (cur == MBG_EMPTY_CELL) cur = key;
+        }
+        if (cur == key) {
+            atomicAdd(D.cell_vals + slot, v);
+            return;
+        }
+    }
+    atomicAdd(D.cell_miss, 1ull);
+}
+#endif
+
+__device__ __forceinline__ void flush_cell(const mbg_desc& D, int w, sidx s, u64 v) {
+#if MBG_SPARSE
+    if (v) sparse_add(D, ((u64)w << D.K) | s, v);
+#else
     if (v) atomicAdd(D.prob + ((u64)w << D.K) + s, v);
+#endif
 }
 
 #define DEP_BATCH MBG_DEP_BATCH
@@ -463,7 +514,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #endif
     int ro_act = 0;          // readouts with a non-zero rate
     double t = 0.0, th = 0.0;
-    u32 s = 0, acc_s = 0;
+    sidx s = 0, acc_s = 0;
     int cur_w = -1;
     bool dirty = true;
     u32 d4[4] = {0u, 0u, 0u, 0u};
@@ -510,7 +561,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     for (int j = 0; j < D.K; j++) {
                         const u32 n = TRO(j);
                         const u32 xi = L.xbit(n);
-                        s |= xi << j;
+                        s |= (sidx)xi << j;
                         const mbg_nd nd = TNODE(n);
                         u64 r;
 #if MBG_GENERIC
@@ -636,7 +687,11 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 flush_cell(D, cur_w, acc_s, acc_p);
                 if (acc_th) atomicAdd(D.th + cur_w, acc_th);
             }
+#if MBG_SPARSE
+            sparse_add(D, ((u64)D.W << D.K) | s, 1ull);
+#else
             atomicAdd(D.fin + s, 1ull);
+#endif
             atomicAdd(D.flip_total, flips);
             if (fixed_pt) {
                 const u64 slot = atomicAdd(D.fp_count, 1ull);
@@ -795,7 +850,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 }
             }
         }
-        if (nj.flags & F_RO) s ^= 1u << ((nj.flags & F_RO) - 1u);
+        if (nj.flags & F_RO) s ^= (sidx)1 << ((nj.flags & F_RO) - 1u);
         flips++;
 #if MBG_TRACE
         hash = (hash ^ (u64)j) * 0x100000001b3ull;

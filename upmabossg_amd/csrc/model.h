@@ -30,7 +30,9 @@ static_assert(sizeof(NodeRec) == 16, "NodeRec must match the device layout");
 
 constexpr int kMaxInputsFast = 20;     // LUT of 2^21 bits
 constexpr int kMaxInputsGeneric = 16;  // table of 2^17 rates
-constexpr int kMaxReadouts = 20;       // src/main.cpp:142-146
+constexpr int kMaxReadouts = 20;       // dense window tables (the reference's limit, src/main.cpp:142-146)
+constexpr int kMaxSparseReadouts = 63; // sparse window tables (hash of window and projected state)
+constexpr const char* kReadoutLimitMessage = "This executable supports a maximum of 63 non-internal nodes.";
 
 struct CompiledModel {
     std::vector<std::string> names;

@@ -10,12 +10,13 @@
 
 namespace mbg {
 
-double window_prob(const Results& r, size_t cell) {
-    const mbg_run_info& I = *r.info;
-    const double v = (double)(*r.prob)[cell];
+double cell_prob(const mbg_run_info& I, uint64_t raw) {
+    const double v = (double)raw;
     if (I.discrete) return v / (double)I.n_trajectories;
     return std::ldexp(v, -I.prob_exponent) / ((double)I.n_trajectories * (double)I.time_tick);
 }
+
+double window_prob(const Results& r, size_t cell) { return cell_prob(*r.info, (*r.prob)[cell]); }
 
 double window_th(const Results& r, size_t w) {
     const mbg_run_info& I = *r.info;
@@ -52,16 +53,24 @@ struct WindowRow {
 };
 
 WindowRow window_row(const Results& r, size_t w) {
-    const uint64_t S = 1ull << r.info->readouts;
     WindowRow row;
     row.time = (float)w * r.info->time_tick;
     row.th = (float)window_th(r, w);
     row.entropy = 0.f;
-    for (uint64_t s = 0; s < S; s++) {
-        const float p = (float)window_prob(r, w * S + s);
-        if (p == 0.f) continue;
+    auto add = [&](uint64_t s, uint64_t raw) {
+        const float p = (float)cell_prob(*r.info, raw);
+        if (p == 0.f) return;
         row.entropy += -std::log2(p) * p;
         row.probs.emplace_back(s, p);
+    };
+    if (r.cell_w) {
+        const auto lo = std::lower_bound(r.cell_w->begin(), r.cell_w->end(), (uint32_t)w);
+        const auto hi = std::upper_bound(lo, r.cell_w->end(), (uint32_t)w);
+        for (size_t i = (size_t)(lo - r.cell_w->begin()); i < (size_t)(hi - r.cell_w->begin()); i++)
+            add((*r.cell_s)[i], (*r.cell_v)[i]);
+    } else {
+        const uint64_t S = 1ull << r.info->readouts;
+        for (uint64_t s = 0; s < S; s++) add(s, (*r.prob)[w * S + s]);
     }
     return row;
 }
@@ -110,8 +119,13 @@ bool write_fp_csv(const Results& r, const CompiledModel& cm, const std::string& 
 void write_text(const Results& r, const CompiledModel& cm, std::ostream& os) {
     const float n = (float)r.info->n_trajectories;
     os << "final points:" << std::endl;
-    for (size_t s = 0; s < r.fin->size(); s++)
-        if ((*r.fin)[s]) os << (float)(*r.fin)[s] / n << " " << projected_string(cm, s) << std::endl;
+    if (r.fin_s) {
+        for (size_t i = 0; i < r.fin_s->size(); i++)
+            os << (float)(*r.fin_v)[i] / n << " " << projected_string(cm, (*r.fin_s)[i]) << std::endl;
+    } else {
+        for (size_t s = 0; s < r.fin->size(); s++)
+            if ((*r.fin)[s]) os << (float)(*r.fin)[s] / n << " " << projected_string(cm, s) << std::endl;
+    }
     os << "fixed points:" << std::endl;
     for (size_t i = 0; i < r.fixed_counts->size(); i++)
         os << (float)(*r.fixed_counts)[i] / n << " " << state_string(cm, r.fixed_states->data() + i * cm.NW) << std::endl;

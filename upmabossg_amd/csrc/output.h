@@ -32,7 +32,17 @@ struct Results {
     const std::vector<uint64_t>* fin;           // [2^K]
     const std::vector<uint64_t>* fixed_states;  // [F][state_words], reference order
     const std::vector<uint64_t>* fixed_counts;  // [F]
+    // More than 20 readouts (prob and fin empty): the non-zero window cells, window-major with
+    // states ascending, and the final states, ascending.
+    const std::vector<uint32_t>* cell_w = nullptr;
+    const std::vector<uint64_t>* cell_s = nullptr;
+    const std::vector<uint64_t>* cell_v = nullptr;
+    const std::vector<uint64_t>* fin_s = nullptr;
+    const std::vector<uint64_t>* fin_v = nullptr;
 };
+
+// Normalised probability of a raw cell value.
+double cell_prob(const mbg_run_info& I, uint64_t v);
 
 // Normalised probability of cell (window-major) and window transition entropy
 // (src/statistics/window_average_small.cpp:126-140, :222-223).

@@ -93,18 +93,22 @@ struct Desc {
     uint32_t variant_id;
     int N, K, W, FP, FT, sigma;
     int compat, pad_;
+    void* cells;       // sparse window table (more than 20 readouts): keys, values, misses
+    void* cell_vals;
+    void* cell_miss;
+    uint64_t cell_mask;
 };
 
 using mbg_desc_ptr = const void*;
 
 struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 2, scan = 4, wsum_reg = 0, rate32 = 0,
-        multi = 0, sumw = 1, compact = 0, wsmad = 0;
+        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
-                        sumw, compact, wsmad) < std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds,
-                                                         o.max_in, o.dep_batch, o.scan, o.wsum_reg, o.rate32, o.multi,
-                                                         o.sumw, o.compact, o.wsmad);
+                        sumw, compact, wsmad, sparse) < std::tie(o.nw, o.block, o.discrete, o.trace, o.generic,
+                                                                 o.tables_lds, o.max_in, o.dep_batch, o.scan, o.wsum_reg,
+                                                                 o.rate32, o.multi, o.sumw, o.compact, o.wsmad, o.sparse);
     }
 };
 
@@ -127,6 +131,7 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_SUMW " << sh.sumw << "\n";
     os << "#define MBG_COMPACT " << sh.compact << "\n";
     os << "#define MBG_WSMAD " << sh.wsmad << "\n";
+    os << "#define MBG_SPARSE " << sh.sparse << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -248,14 +253,15 @@ struct DevBuf {
         alloc(v.size());
         if (!v.empty()) hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
     }
-    void zero(size_t count, hipStream_t s) {
+    void zero(size_t count, hipStream_t s) { fill(count, 0, s); }
+    void fill(size_t count, int byte, hipStream_t s) {
         alloc(count);
-        hip_check(hipMemsetAsync(p, 0, std::max<size_t>(1, count) * sizeof(T), s), "hipMemsetAsync");
+        hip_check(hipMemsetAsync(p, byte, std::max<size_t>(1, count) * sizeof(T), s), "hipMemsetAsync");
     }
 };
 
 struct RunBuffers {
-    DevBuf<uint64_t> blob, rtab, stage, prob, th, fin, fp, fpc, summ, work, flips;
+    DevBuf<uint64_t> blob, rtab, stage, prob, th, fin, fp, fpc, summ, work, flips, cells, cell_vals, cell_miss;
     DevBuf<float> istate;
 };
 
@@ -328,6 +334,12 @@ struct mbg_ctx {
     hipFunction_t marg_fn = nullptr;  // mbg_marginals of the last run's module
     DevBuf<uint64_t> marg;
     std::vector<uint64_t> th, fin, fixed_states, fixed_counts, summ;
+    // more than 20 readouts: non-zero window cells (window-major, states ascending) and final
+    // states, read back from the device hash table; cell_cap is the table size to start from
+    bool sparse = false;
+    std::vector<uint32_t> cell_w;
+    std::vector<uint64_t> cell_s, cell_v, fin_s, fin_v;
+    uint64_t cell_cap = 0;
 };
 
 namespace {
@@ -459,6 +471,16 @@ void launch(hipFunction_t f, uint64_t grid, int block, size_t smem, hipStream_t 
     hip_check(hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, (unsigned)smem, s, args, nullptr), what);
 }
 
+// Every device buffer a descriptor names must be allocated before a launch reads it.
+void check_desc(const Desc& D, bool sparse) {
+    const void* need[] = {D.blob, D.istate, D.stage, D.prob, D.th, D.fin, D.fp_states, D.fp_count, D.summ,
+                          D.work_counter, D.flip_total};
+    for (const void* q : need)
+        if (!q) throw Error(MBG_ERR_ARG, "internal error: a device buffer of the run is not allocated");
+    if (sparse && (!D.cells || !D.cell_vals || !D.cell_miss))
+        throw Error(MBG_ERR_ARG, "internal error: the sparse window table is not allocated");
+}
+
 // Window count, fixed-point exponents and discrete step count of a run of n trajectories.
 struct RunScales {
     int W = 0, FP = 0, FT = 0;
@@ -580,6 +602,44 @@ std::vector<long long> tune_key(const mbg::CompiledModel& cm, const mbg::RateTab
             (long long)cm.deps.size(), (long long)cm.inputs.size(), (long long)cm.lut.size()};
 }
 
+const char* const kSparseMessage =
+    "the run has more than 20 readouts: its window cells and final states are sparse "
+    "(mbg_get_probtraj_cells, mbg_get_final_cells)";
+
+// Largest sparse window table: 2^30 slots (16 GiB of keys and values).
+constexpr uint64_t kMaxCells = 1ull << 30;
+
+// Reads the sparse table back: the non-empty slots sorted by key (window-major, states
+// ascending), the final states from the keys of window W.  The next sparse run starts from
+// twice the size when more than half the slots were used (short probe sequences).
+void read_cells(mbg_ctx* c, uint64_t cap, int W, int K) {
+    std::vector<uint64_t> keys(cap), vals(cap);
+    hip_check(hipMemcpy(keys.data(), c->buf.cells.p, cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    hip_check(hipMemcpy(vals.data(), c->buf.cell_vals.p, cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    std::vector<std::pair<uint64_t, uint64_t>> kv;
+    for (uint64_t i = 0; i < cap; i++)
+        if (keys[i] != ~0ull) kv.emplace_back(keys[i], vals[i]);
+    std::sort(kv.begin(), kv.end());
+    c->cell_w.clear();
+    c->cell_s.clear();
+    c->cell_v.clear();
+    c->fin_s.clear();
+    c->fin_v.clear();
+    const uint64_t smask = (1ull << K) - 1;
+    for (const auto& e : kv) {
+        const uint64_t w = e.first >> K, st = e.first & smask;
+        if (w == (uint64_t)W) {
+            c->fin_s.push_back(st);
+            c->fin_v.push_back(e.second);
+        } else {
+            c->cell_w.push_back((uint32_t)w);
+            c->cell_s.push_back(st);
+            c->cell_v.push_back(e.second);
+        }
+    }
+    c->cell_cap = kv.size() * 2 > cap ? std::min(cap * 2, kMaxCells) : cap;
+}
+
 // The shape a run uses without measurement: occupancy model plus tuning knobs.
 Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::RateTables& rt, int block_req, int trace) {
     const bool rate32 = rates_fit_u32(rt);
@@ -587,13 +647,18 @@ Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::R
     Shape sh = choose_shape(c, cm, pack_blob(cm, rt, rate32, compact).lay.bytes, block_req, trace, rate32 ? 1 : 0);
     sh.compact = compact ? 1 : 0;
     sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
+    sh.sparse = cm.K > mbg::kMaxReadouts ? 1 : 0;
     return sh;
 }
 
 void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Shape* force = nullptr) {
     const mbg::CompiledModel& cm = m->cm;
-    if (cm.K > mbg::kMaxReadouts) throw Error(MBG_ERR_LIMIT, "This executable supports a maximum of 20 non-internal nodes.");
+    if (cm.K > mbg::kMaxSparseReadouts) throw Error(MBG_ERR_LIMIT, mbg::kReadoutLimitMessage);
     if (!(cm.time_tick > 0.f) || !(cm.max_time > 0.f)) throw Error(MBG_ERR_ARG, "max_time and time_tick must be positive");
+    const bool sparse = cm.K > mbg::kMaxReadouts;
+    if (sparse && mbg::ceil_log2_u64((uint64_t)std::ceil(cm.max_time / cm.time_tick) + 1) + cm.K > 63)
+        throw Error(MBG_ERR_LIMIT, "too many non-internal nodes for this number of windows (readouts + log2(windows + 1) "
+                                   "must stay within 63 bits)");
     const uint64_t n = p->n_trajectories ? p->n_trajectories : (uint64_t)cm.sample_count;
     const uint64_t count = p->count ? p->count : n;
     if (n == 0 || count == 0) throw Error(MBG_ERR_ARG, "no trajectories to simulate");
@@ -610,6 +675,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
         const auto tu = c->tuned.find(tune_key(cm, rt, p->trace));
         sh = (!p->block && tu != c->tuned.end()) ? tu->second : default_shape(c, cm, rt, p->block, p->trace);
     }
+    sh.sparse = sparse ? 1 : 0;
     const Blob blob = pack_blob(cm, rt, sh.rate32 != 0, sh.compact != 0);
     if ((size_t)lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0) >
         (size_t)c->lds_per_block)
@@ -622,7 +688,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
         blocks_per_cu = 1;
 
     const int W = (int)std::ceil(cm.max_time / cm.time_tick);  // window_average_small.cpp:26
-    const uint64_t S = 1ull << cm.K;
+    const uint64_t S = sparse ? 0 : 1ull << cm.K;  // dense table only up to 20 readouts
     int FP, FT;
     if (cm.discrete) {
         FP = 0;
@@ -657,29 +723,26 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     d_rtab.upload(rt.rtab, s);
     d_istate.upload(rt.istate, s);
     d_stage.alloc((size_t)batch_max * 3 * cm.NW);
-    d_prob.zero((size_t)W * S, s);
-    d_th.zero((size_t)W, s);
-    d_fin.zero((size_t)S, s);
     d_fp.alloc((size_t)fp_cap * cm.NW);
-    d_fpc.zero(1, s);
     const uint64_t summ_cap = p->trace ? count : 0;
-    d_summ.zero((size_t)summ_cap * 4, s);
     d_work.alloc(1);
-    d_flips.zero(1, s);
+    // Sparse table: one slot per distinct (window, projected state) cell and final state,
+    // started at the size the last sparse run needed (at least 2^16, about 4 cells per
+    // window per 256 trajectories) and grown four-fold whenever an insert finds no slot.
+    uint64_t cell_cap = 0;
+    if (sparse) {
+        const uint64_t guess = 4ull * (uint64_t)(W + 1) * std::min<uint64_t>(count, 256);
+        cell_cap = std::max<uint64_t>({1ull << 16, c->cell_cap, 1ull << mbg::ceil_log2_u64(guess)});
+        cell_cap = std::min<uint64_t>(cell_cap, kMaxCells);
+    }
 
     Desc D{};
     D.blob = d_blob.p;
     D.rtab = d_rtab.p;
     D.istate = d_istate.p;
     D.stage = d_stage.p;
-    D.prob = d_prob.p;
-    D.th = d_th.p;
-    D.fin = d_fin.p;
     D.fp_states = d_fp.p;
-    D.fp_count = d_fpc.p;
-    D.summ = d_summ.p;
     D.work_counter = d_work.p;
-    D.flip_total = d_flips.p;
     D.fp_cap = fp_cap;
     D.summ_cap = summ_cap;
     D.nsteps = nsteps;
@@ -703,24 +766,60 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
     float sim_ms = 0.f;
-    for (uint64_t done = 0; done < count; done += D.batch) {
-        D.batch = std::min(batch_max, count - done);
-        D.batch_first = p->first_id + done;
-        hip_check(hipMemsetAsync(d_work.p, 0, 8, s), "hipMemsetAsync");
-        const uint64_t init_threads = D.batch * (uint64_t)cm.NW;
-        // mbg_init_rates: one block per 64 trajectories (per (trajectory, word) above 128 words)
-        const uint64_t rate_blocks = cm.NW <= 128 ? (D.batch + 63) / 64 : (init_threads + 255) / 256;
-        if (cm.NW > 128) launch(mod.init_state, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_state");
-        launch(mod.init_rates, rate_blocks, 256, 0, s, D, "launch mbg_init_rates");
-        uint64_t grid = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)blocks_per_cu;
-        grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (D.batch + sh.block - 1) / sh.block));
-        hip_check(hipEventRecord(e0, s), "hipEventRecord");
-        launch(mod.sim, grid, sh.block, smem, s, D, "launch mbg_simulate");
-        hip_check(hipEventRecord(e1, s), "hipEventRecord");
-        hip_check(hipStreamSynchronize(s), "kernel execution");
-        float ms = 0.f;
-        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-        sim_ms += ms;
+    for (;;) {
+        d_prob.zero((size_t)W * S, s);
+        d_th.zero((size_t)W, s);
+        d_fin.zero((size_t)S, s);
+        d_fpc.zero(1, s);
+        d_summ.zero((size_t)summ_cap * 4, s);
+        d_flips.zero(1, s);
+        if (sparse) {
+            B.cells.fill(cell_cap, 0xff, s);  // every key ~0: empty
+            B.cell_vals.zero(cell_cap, s);
+            B.cell_miss.zero(1, s);
+        }
+        D.prob = d_prob.p;
+        D.th = d_th.p;
+        D.fin = d_fin.p;
+        D.fp_count = d_fpc.p;
+        D.summ = d_summ.p;
+        D.flip_total = d_flips.p;
+        D.cells = sparse ? B.cells.p : nullptr;
+        D.cell_vals = sparse ? B.cell_vals.p : nullptr;
+        D.cell_miss = sparse ? B.cell_miss.p : nullptr;
+        D.cell_mask = sparse ? cell_cap - 1 : 0;
+        check_desc(D, sparse);
+        sim_ms = 0.f;
+        for (uint64_t done = 0; done < count; done += D.batch) {
+            D.batch = std::min(batch_max, count - done);
+            D.batch_first = p->first_id + done;
+            hip_check(hipMemsetAsync(d_work.p, 0, 8, s), "hipMemsetAsync");
+            const uint64_t init_threads = D.batch * (uint64_t)cm.NW;
+            // mbg_init_rates: one block per 64 trajectories (per (trajectory, word) above 128 words)
+            const uint64_t rate_blocks = cm.NW <= 128 ? (D.batch + 63) / 64 : (init_threads + 255) / 256;
+            if (cm.NW > 128) launch(mod.init_state, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_state");
+            launch(mod.init_rates, rate_blocks, 256, 0, s, D, "launch mbg_init_rates");
+            uint64_t grid = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)blocks_per_cu;
+            grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (D.batch + sh.block - 1) / sh.block));
+            hip_check(hipEventRecord(e0, s), "hipEventRecord");
+            launch(mod.sim, grid, sh.block, smem, s, D, "launch mbg_simulate");
+            hip_check(hipEventRecord(e1, s), "hipEventRecord");
+            hip_check(hipStreamSynchronize(s), "kernel execution");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            sim_ms += ms;
+        }
+        if (!sparse) break;
+        uint64_t miss = 0;
+        hip_check(hipMemcpy(&miss, B.cell_miss.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (miss == 0) break;
+        if (cell_cap >= kMaxCells) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw Error(MBG_ERR_LIMIT, "sparse window table full: more distinct (window, state) cells than " +
+                                           std::to_string(kMaxCells / 2));
+        }
+        cell_cap = std::min<uint64_t>(cell_cap * 4, kMaxCells);  // repeat the run with a larger table
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -730,7 +829,9 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     c->th.resize(W);
     c->fin.resize(S);
     hip_check(hipMemcpy(c->th.data(), d_th.p, c->th.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-    hip_check(hipMemcpy(c->fin.data(), d_fin.p, c->fin.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (S) hip_check(hipMemcpy(c->fin.data(), d_fin.p, c->fin.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    c->sparse = sparse;
+    if (sparse) read_cells(c, cell_cap, W, cm.K);
     uint64_t nfp = 0, flips = 0;
     hip_check(hipMemcpy(&nfp, d_fpc.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
     hip_check(hipMemcpy(&flips, d_flips.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
@@ -900,7 +1001,8 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         const mbg_model* m = vs[v].model;
         if (!m) throw Error(MBG_ERR_ARG, "variant without a model");
         const mbg::CompiledModel& cm = m->cm;
-        if (cm.K > mbg::kMaxReadouts) throw Error(MBG_ERR_LIMIT, "This executable supports a maximum of 20 non-internal nodes.");
+        if (cm.K > mbg::kMaxReadouts)
+            throw Error(MBG_ERR_LIMIT, "variant sweeps support at most 20 non-internal nodes (run the variant alone)");
         if (!(cm.time_tick > 0.f) || !(cm.max_time > 0.f)) throw Error(MBG_ERR_ARG, "max_time and time_tick must be positive");
         std::unique_ptr<VariantRun> V;
         if (!c->variant_pool.empty()) {
@@ -1234,6 +1336,13 @@ mbg::Results results_view(const mbg_ctx* c) {
     r.prob = &c->prob;
     r.th = &c->th;
     r.fin = &c->fin;
+    if (c->sparse) {
+        r.cell_w = &c->cell_w;
+        r.cell_s = &c->cell_s;
+        r.cell_v = &c->cell_v;
+        r.fin_s = &c->fin_s;
+        r.fin_v = &c->fin_v;
+    }
     r.fixed_states = &c->fixed_states;
     r.fixed_counts = &c->fixed_counts;
     return r;
@@ -1282,8 +1391,8 @@ mbg_status parse_impl(const std::string& bnd, const std::string& cfg, mbg_model*
         copy_err(err, errlen, e.what());
         return MBG_ERR_PARSE;
     }
-    if (m->cm.K > mbg::kMaxReadouts) {
-        copy_err(err, errlen, "This executable supports a maximum of 20 non-internal nodes.");
+    if (m->cm.K > mbg::kMaxSparseReadouts) {
+        copy_err(err, errlen, mbg::kReadoutLimitMessage);
         return MBG_ERR_LIMIT;
     }
     *out = m.release();
@@ -1538,6 +1647,7 @@ mbg_status mbg_get_probtraj_raw(const mbg_ctx* c, uint64_t* prob, uint64_t* th) 
     if (!c) return MBG_ERR_ARG;
     if (!c->have) return MBG_ERR_NOT_BUILT;
     return guarded(const_cast<mbg_ctx*>(c), [&] {
+        if (prob && c->sparse) throw Error(MBG_ERR_LIMIT, kSparseMessage);
         if (prob) {
             ensure_prob(c);
             std::memcpy(prob, c->prob.data(), c->prob.size() * 8);
@@ -1550,6 +1660,7 @@ mbg_status mbg_get_probtraj(const mbg_ctx* c, double* prob, double* th) {
     if (!c) return MBG_ERR_ARG;
     if (!c->have) return MBG_ERR_NOT_BUILT;
     return guarded(const_cast<mbg_ctx*>(c), [&] {
+        if (prob && c->sparse) throw Error(MBG_ERR_LIMIT, kSparseMessage);
         const mbg::Results r = results_view(c);
         if (prob)
             for (size_t i = 0; i < c->prob.size(); i++) prob[i] = mbg::window_prob(r, i);
@@ -1561,7 +1672,58 @@ mbg_status mbg_get_probtraj(const mbg_ctx* c, double* prob, double* th) {
 mbg_status mbg_get_final_states(const mbg_ctx* c, uint64_t* counts) {
     if (!c || !counts) return MBG_ERR_ARG;
     if (!c->have) return MBG_ERR_NOT_BUILT;
+    if (c->sparse) {
+        const_cast<mbg_ctx*>(c)->err = kSparseMessage;
+        return MBG_ERR_LIMIT;
+    }
     std::memcpy(counts, c->fin.data(), c->fin.size() * 8);
+    return MBG_OK;
+}
+
+mbg_status mbg_get_probtraj_cells(const mbg_ctx* c, uint32_t* windows, uint64_t* states, uint64_t* cells, size_t cap,
+                                  size_t* n) {
+    if (!c) return MBG_ERR_ARG;
+    if (!c->have) return MBG_ERR_NOT_BUILT;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        size_t k = 0;
+        auto put = [&](uint32_t w, uint64_t st, uint64_t v) {
+            if (k < cap) {
+                if (windows) windows[k] = w;
+                if (states) states[k] = st;
+                if (cells) cells[k] = v;
+            }
+            k++;
+        };
+        if (c->sparse) {
+            for (size_t i = 0; i < c->cell_v.size(); i++) put(c->cell_w[i], c->cell_s[i], c->cell_v[i]);
+        } else {
+            ensure_prob(c);
+            const uint64_t S = 1ull << c->info.readouts;
+            for (size_t i = 0; i < c->prob.size(); i++)
+                if (c->prob[i]) put((uint32_t)(i / S), i % S, c->prob[i]);
+        }
+        if (n) *n = k;
+    });
+}
+
+mbg_status mbg_get_final_cells(const mbg_ctx* c, uint64_t* states, uint64_t* counts, size_t cap, size_t* n) {
+    if (!c) return MBG_ERR_ARG;
+    if (!c->have) return MBG_ERR_NOT_BUILT;
+    size_t k = 0;
+    auto put = [&](uint64_t st, uint64_t v) {
+        if (k < cap) {
+            if (states) states[k] = st;
+            if (counts) counts[k] = v;
+        }
+        k++;
+    };
+    if (c->sparse) {
+        for (size_t i = 0; i < c->fin_v.size(); i++) put(c->fin_s[i], c->fin_v[i]);
+    } else {
+        for (size_t s = 0; s < c->fin.size(); s++)
+            if (c->fin[s]) put(s, c->fin[s]);
+    }
+    if (n) *n = k;
     return MBG_OK;
 }
 
@@ -1621,7 +1783,11 @@ mbg_status mbg_get_marginals(const mbg_ctx* c, double* out) {
         mbg_ctx* cc = const_cast<mbg_ctx*>(c);
         const int W = c->info.windows, K = c->info.readouts;
         std::vector<uint64_t> raw((size_t)W * K, 0);
-        if (K > 0) {
+        if (c->sparse) {
+            for (size_t i = 0; i < c->cell_v.size(); i++)
+                for (int j = 0; j < K; j++)
+                    if ((c->cell_s[i] >> j) & 1ull) raw[(size_t)c->cell_w[i] * K + j] += c->cell_v[i];
+        } else if (K > 0) {
             // reduced on the device from the window table (no transfer of the whole table)
             hip_check(hipSetDevice(c->device), "hipSetDevice");
             cc->marg.alloc(raw.size());
