@@ -997,3 +997,47 @@ extern "C" __global__ __launch_bounds__(256) void mbg_fp_gather(const u64* state
     const int w = (int)(t - j * (u64)words);
     out[t] = states[(u64)idx[j] * (u64)stride + w];
 }
+
+// ------------------------------------------------------------------ sparse table read-back
+// Taken slots (key != ~0) of the sparse cell table: mbg_cells_count counts them, then
+// mbg_cells_compact copies them to dense arrays, in no particular order (the host sorts).
+// Both reduce per block (wave ballots, then LDS) and issue one atomic per block.
+__device__ __forceinline__ u32 block_exclusive_count(bool has, u32* lds, u32& total) {
+    const u32 tid = threadIdx.x, lid = tid & 63u, wid = tid >> 6;
+    const u64 mask = __ballot(has);
+    if (lid == 0) lds[wid] = (u32)__popcll(mask);
+    __syncthreads();
+    u32 before = 0;
+    total = 0;
+    for (u32 k = 0; k < 4u; k++) {
+        before += k < wid ? lds[k] : 0u;
+        total += lds[k];
+    }
+    return before + (u32)__popcll(mask & ((1ull << lid) - 1ull));
+}
+
+extern "C" __global__ void __launch_bounds__(256) mbg_cells_count(const u64* keys, u64 cap, u64* out_n) {
+    __shared__ u32 lds[4];
+    const u64 i = (u64)blockIdx.x * 256u + threadIdx.x;
+    u32 total;
+    (void)block_exclusive_count(i < cap && keys[i] != ~0ull, lds, total);
+    if (threadIdx.x == 0 && total) atomicAdd(out_n, (u64)total);
+}
+
+extern "C" __global__ void __launch_bounds__(256) mbg_cells_compact(const u64* keys, const u64* vals, u64 cap,
+                                                                    u64* out_keys, u64* out_vals, u64* out_n,
+                                                                    u64 out_cap) {
+    __shared__ u32 lds[4];
+    __shared__ u64 base;
+    const u64 i = (u64)blockIdx.x * 256u + threadIdx.x;
+    const bool has = i < cap && keys[i] != ~0ull;
+    u32 total;
+    const u32 rank = block_exclusive_count(has, lds, total);
+    if (threadIdx.x == 0) base = total ? atomicAdd(out_n, (u64)total) : 0ull;
+    __syncthreads();
+    const u64 pos = base + rank;
+    if (has && pos < out_cap) {  // out_n above out_cap is reported to the host as an error
+        out_keys[pos] = keys[i];
+        out_vals[pos] = vals[i];
+    }
+}

@@ -17,6 +17,7 @@
 #include <numeric>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -226,7 +227,7 @@ struct Module {
     hipModule_t mod = nullptr;
     hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr,
                   marginals_batch = nullptr, init_rates_multi = nullptr, fp_dedup = nullptr, fp_compact = nullptr,
-                  fp_gather = nullptr;
+                  fp_gather = nullptr, cells_count = nullptr, cells_compact = nullptr;
 };
 
 // Device buffer kept by the context across runs; it only grows.
@@ -261,7 +262,7 @@ struct DevBuf {
 };
 
 struct RunBuffers {
-    DevBuf<uint64_t> blob, rtab, stage, prob, th, fin, fp, fpc, summ, work, flips, cells, cell_vals, cell_miss;
+    DevBuf<uint64_t> blob, rtab, stage, prob, th, fin, fp, fpc, summ, work, flips, cells, cell_vals, cell_miss, cell_out;
     DevBuf<float> istate;
 };
 
@@ -376,6 +377,8 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
     hip_check(hipModuleGetFunction(&m.fp_dedup, m.mod, "mbg_fp_dedup"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.fp_compact, m.mod, "mbg_fp_compact"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.fp_gather, m.mod, "mbg_fp_gather"), "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.cells_count, m.mod, "mbg_cells_count"), "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.cells_compact, m.mod, "mbg_cells_compact"), "hipModuleGetFunction");
     if (sh.multi && sh.nw <= 128)
         hip_check(hipModuleGetFunction(&m.init_rates_multi, m.mod, "mbg_init_rates_multi"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_state, m.mod, "mbg_init_state"), "hipModuleGetFunction");
@@ -609,35 +612,81 @@ const char* const kSparseMessage =
 // Largest sparse window table: 2^30 slots (16 GiB of keys and values).
 constexpr uint64_t kMaxCells = 1ull << 30;
 
-// Reads the sparse table back: the non-empty slots sorted by key (window-major, states
-// ascending), the final states from the keys of window W.  The next sparse run starts from
-// twice the size when more than half the slots were used (short probe sequences).
-void read_cells(mbg_ctx* c, uint64_t cap, int W, int K) {
-    std::vector<uint64_t> keys(cap), vals(cap);
-    hip_check(hipMemcpy(keys.data(), c->buf.cells.p, cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-    hip_check(hipMemcpy(vals.data(), c->buf.cell_vals.p, cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-    std::vector<std::pair<uint64_t, uint64_t>> kv;
-    for (uint64_t i = 0; i < cap; i++)
-        if (keys[i] != ~0ull) kv.emplace_back(keys[i], vals[i]);
-    std::sort(kv.begin(), kv.end());
-    c->cell_w.clear();
-    c->cell_s.clear();
-    c->cell_v.clear();
-    c->fin_s.clear();
-    c->fin_v.clear();
-    const uint64_t smask = (1ull << K) - 1;
-    for (const auto& e : kv) {
-        const uint64_t w = e.first >> K, st = e.first & smask;
-        if (w == (uint64_t)W) {
-            c->fin_s.push_back(st);
-            c->fin_v.push_back(e.second);
-        } else {
-            c->cell_w.push_back((uint32_t)w);
-            c->cell_s.push_back(st);
-            c->cell_v.push_back(e.second);
-        }
+// Reads the sparse table back: the taken slots are counted and compacted on the device
+// (mbg_cells_count, mbg_cells_compact) and copied, then sorted by key (window-major, states ascending) with a counting sort on the
+// window followed by per-window sorts on up to 16 host threads; the keys of window W are the
+// final states.  The next sparse run starts from twice the size when more than half the slots
+// were used (short probe sequences).
+void read_cells(mbg_ctx* c, const Module& mod, uint64_t cap, int W, int K, hipStream_t s) {
+    RunBuffers& B = c->buf;
+    const unsigned blocks = (unsigned)((cap + 255) / 256);
+    uint64_t n = 0;
+    {
+        B.cell_out.alloc(1);
+        hip_check(hipMemsetAsync(B.cell_out.p, 0, 8, s), "hipMemsetAsync");
+        const void* kp = B.cells.p;
+        uint64_t capv = cap;
+        void* on = B.cell_out.p;
+        void* args[] = {(void*)&kp, &capv, &on};
+        hip_check(hipModuleLaunchKernel(mod.cells_count, blocks, 1, 1, 256, 1, 1, 0, s, args, nullptr),
+                  "launch mbg_cells_count");
+        hip_check(hipMemcpyAsync(&n, B.cell_out.p, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(s), "mbg_cells_count");
     }
-    c->cell_cap = kv.size() * 2 > cap ? std::min(cap * 2, kMaxCells) : cap;
+    if (n > cap) throw Error(MBG_ERR_HIP, "sparse window table: inconsistent slot count");
+    std::vector<uint64_t> keys(n), vals(n);
+    if (n) {
+        B.cell_out.alloc(2 * n + 1);
+        hip_check(hipMemsetAsync(B.cell_out.p + 2 * n, 0, 8, s), "hipMemsetAsync");
+        const void* kp = B.cells.p;
+        const void* vp = B.cell_vals.p;
+        void* ok = B.cell_out.p;
+        void* ov = B.cell_out.p + n;
+        void* on = B.cell_out.p + 2 * n;
+        uint64_t capv = cap, outv = n;
+        void* args[] = {(void*)&kp, (void*)&vp, &capv, &ok, &ov, &on, &outv};
+        hip_check(hipModuleLaunchKernel(mod.cells_compact, blocks, 1, 1, 256, 1, 1, 0, s, args, nullptr),
+                  "launch mbg_cells_compact");
+        uint64_t got = 0;
+        hip_check(hipMemcpyAsync(&got, on, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(keys.data(), ok, n * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(vals.data(), ov, n * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(s), "mbg_cells_compact");
+        if (got != n) throw Error(MBG_ERR_HIP, "sparse window table: compaction count mismatch");
+    }
+    // counting sort on the window (W + 1 buckets), then each bucket by key
+    std::vector<uint64_t> off((size_t)W + 2, 0);
+    for (uint64_t i = 0; i < n; i++) off[(keys[i] >> K) + 1]++;
+    for (int b = 0; b <= W; b++) off[b + 1] += off[b];
+    std::vector<std::pair<uint64_t, uint64_t>> kv(n);
+    {
+        std::vector<uint64_t> pos(off.begin(), off.end() - 1);
+        for (uint64_t i = 0; i < n; i++) kv[pos[keys[i] >> K]++] = {keys[i], vals[i]};
+    }
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+        pool.emplace_back([&, t] {
+            for (int b = t; b <= W; b += nt) std::sort(kv.begin() + (long)off[b], kv.begin() + (long)off[b + 1]);
+        });
+    for (auto& th : pool) th.join();
+    const uint64_t smask = (1ull << K) - 1;
+    const uint64_t nf = off[(size_t)W + 1] - off[(size_t)W], nc = n - nf;
+    c->cell_w.resize(nc);
+    c->cell_s.resize(nc);
+    c->cell_v.resize(nc);
+    c->fin_s.resize(nf);
+    c->fin_v.resize(nf);
+    for (uint64_t i = 0; i < nc; i++) {
+        c->cell_w[i] = (uint32_t)(kv[i].first >> K);
+        c->cell_s[i] = kv[i].first & smask;
+        c->cell_v[i] = kv[i].second;
+    }
+    for (uint64_t i = 0; i < nf; i++) {
+        c->fin_s[i] = kv[nc + i].first & smask;
+        c->fin_v[i] = kv[nc + i].second;
+    }
+    c->cell_cap = n * 2 > cap ? std::min(cap * 2, kMaxCells) : cap;
 }
 
 // The shape a run uses without measurement: occupancy model plus tuning knobs.
@@ -831,7 +880,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     hip_check(hipMemcpy(c->th.data(), d_th.p, c->th.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     if (S) hip_check(hipMemcpy(c->fin.data(), d_fin.p, c->fin.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     c->sparse = sparse;
-    if (sparse) read_cells(c, cell_cap, W, cm.K);
+    if (sparse) read_cells(c, mod, cell_cap, W, cm.K, s);
     uint64_t nfp = 0, flips = 0;
     hip_check(hipMemcpy(&nfp, d_fpc.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
     hip_check(hipMemcpy(&flips, d_flips.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
