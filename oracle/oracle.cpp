@@ -521,9 +521,8 @@ struct Compiler {
     const Driver& d;
     int self;
     std::set<int>* refs;
-    int depth = 0;
+    int depth = 0;  // nested @alias references (a self-referencing alias never ends)
     void emit(const Expr* e, Prog& pr) {
-        if (++depth > 64) throw Error("alias recursion too deep");
         switch (e->k) {
             case Expr::LIT: pr.push_back({P_CONST, 0, e->val}); break;
             case Expr::VAR: pr.push_back({P_CONST, 0, eval_const(e, d)}); break;
@@ -538,7 +537,9 @@ struct Compiler {
             case Expr::ALIAS: {  // expressions.cpp:223-231
                 const Expr* a = d.nodes[self].attr(e->name.substr(1));
                 if (!a) throw Error("Node " + d.nodes[self].name + " does not have attribute " + e->name.substr(1));
+                if (++depth > 64) throw Error("alias recursion too deep");
                 emit(a, pr);
+                depth--;
                 break;
             }
             case Expr::UN:
@@ -561,7 +562,6 @@ struct Compiler {
                 break;
             }
         }
-        depth--;
     }
 };
 

@@ -124,10 +124,18 @@ CASES = {
     "metastasis": lambda: model_text("metastasis"),  # continuous, 1,000 trajectories, fixed points
     "cellcycle": lambda: (model_text("cellcycle")[0], model_text("cellcycle")[1] + "\nsample_count = 20000;\n"),
     "readouts24": lambda: _readouts24(),
+    "ring100": lambda: _ring100(),
     "frozen": lambda: ("node A { logic = A; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n"
                        "node B { logic = !A; rate_up = @logic ? 2 : 0; rate_down = @logic ? 0 : 2; }\n",
                        "A.istate = 0.5; B.istate = 0.5; max_time = 5; time_tick = 1; sample_count = 1000;"),
 }
+
+
+def _ring100():
+    # a ring model of the reference's generator (21 inputs per node: logic programs)
+    from upmabossg_amd import synth
+
+    return synth.ring_network(100, steps_to_fixed_point=50, sample_count=3000)
 
 
 def _readouts24():

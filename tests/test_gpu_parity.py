@@ -274,3 +274,26 @@ def test_more_than_20_readouts_bit_exact(gpu_ctx, n, readouts, discrete, count):
         np.add.at(raw[:, j], w[on].astype(np.int64), v[on].astype(np.float64))
     want = raw / count if discrete else raw / (count * float(np.float32(net.time_tick)) * 2.0 ** g.prob_exponent)
     assert np.allclose(gpu_ctx.marginals(), want, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("n,steps,discrete,count", [(100, 0, False, 2000), (100, 50, False, 2000),
+                                                   (100, 20, True, 2000), (1000, 200, False, 256)])
+def test_ring_models_of_the_reference_generator_bit_exact(gpu_ctx, n, steps, discrete, count):
+    """The ring models of data/generate-synth.py: every ring node reads 2 sqrt(N) + 1 other
+    nodes (21 at 100 nodes, 63 at 1,000), so its logic runs as a postfix program."""
+    b, c = synth.ring_network(n, steps_to_fixed_point=steps, sample_count=count, discrete=discrete,
+                              time_tick=5.0, max_time=100.0)
+    g, r = _compare(gpu_ctx, b, c, count=count)
+    assert g.transitions > 10 * count
+    if steps and not discrete:  # (20 discrete steps rarely pick NFP among the active ring nodes)
+        assert g.fixed_point_trajectories > 0
+
+
+@pytest.mark.parametrize("seed,discrete", [(1, False), (2, False), (3, True), (4, False)])
+def test_wide_logic_programs_bit_exact(gpu_ctx, seed, discrete):
+    """Nodes reading 21-40 nodes through random trees of & | ^ ! ?:, constants, $variables,
+    @aliases and unary minus, some reading themselves (tests/wide_models.py)."""
+    from tests.wide_models import wide_model
+
+    b, c = wide_model(seed=seed, discrete=discrete)
+    _compare(gpu_ctx, b, c, count=2000)

@@ -98,6 +98,29 @@ def test_more_than_63_readouts_is_a_limit_error():
     assert Model(net.bnd(), net.cfg()).num_readouts == 63
 
 
+def test_wide_fast_nodes_get_logic_programs():
+    # the ring models of the reference's generator (data/generate-synth.py) read 2 sqrt(N) + 1
+    # other nodes: above 20 a fast node's logic runs as a program instead of a truth table
+    for n, steps in [(100, 0), (1000, 200)]:
+        b, c = synth.ring_network(n, steps_to_fixed_point=steps, sample_count=100)
+        m = Model(b, c)
+        assert m.num_nodes == n
+        assert "#define MBG_PROG 1" in m.generate_source()
+    b, c = synth.config_small().bnd(), synth.config_small().cfg()
+    assert "#define MBG_PROG 0" in Model(b, c).generate_source()
+    ins = " & ".join(f"B{k}" for k in range(21))
+    nodes = "".join(f"node B{k} {{ logic = B{k}; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }}\n"
+                    for k in range(21))
+    # operators that test more than zero-ness are refused for such nodes
+    with pytest.raises(MbgError) as e:
+        Model(nodes + f"node A {{ logic = ({ins}) + B0; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }}", "")
+    assert e.value.status == 1 and "may then only use" in str(e.value)
+    # generic rates keep their 16-input table limit
+    with pytest.raises(MbgError) as e:
+        Model(nodes + f"node A {{ logic = {ins}; rate_up = ({ins}) ? 2 : 1; rate_down = 1; }}", "")
+    assert "limit 16" in str(e.value)
+
+
 def test_grammar_details():
     # keywords are case-insensitive, TRUE/FALSE, comments, precedence of ?: and unary minus
     bnd = ("Node A { logic = B and NOT C Or false; rate_up = @logic ? 2*3-4/2 : 0; rate_down = @logic ? 0 : -(-1.5e0); }\n"

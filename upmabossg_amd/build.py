@@ -5,7 +5,7 @@
 * Embeds the device sources (csrc/device/*) as C strings for hipRTC.
 * Compiles the host library with hipcc (host code only; the trajectory kernel is
   compiled at run time by hipRTC for gfx950) and links the CLI against it.
-* Compile-checks the generated kernel for gfx950 with hipcc for five shapes, so a
+* Compile-checks the generated kernel for gfx950 with hipcc for six shapes, so a
   device-code error shows up at build time rather than at the first run.
 * Compiles the surrogate MLP library (libmbg_mlp.so, BF16 MFMA kernels) for gfx950.
 * Builds the CPU oracle (oracle/liboracle.so, test infrastructure).
@@ -69,7 +69,7 @@ def embed_sources() -> str:
 def kernel_source(nw: int, block: int = 64, discrete: int = 0, trace: int = 0, generic: int = 0,
                   tables_lds: int = 1, max_in: int = 4, dep_batch: int = 2, scan: int = 4, wsum_reg: int = 0,
                   rate32: int = 0, multi: int = 0, sumw: int = 1, compact: int = 0, wsmad: int = 0,
-                  sparse: int = 0) -> str:
+                  sparse: int = 0, prog: int = 0) -> str:
     with open(os.path.join(DEV, "mbg_math.h")) as f:
         math_src = f.read()
     with open(os.path.join(DEV, "mbg_kernel.hip")) as f:
@@ -79,20 +79,24 @@ def kernel_source(nw: int, block: int = 64, discrete: int = 0, trace: int = 0, g
             f"#define MBG_MAX_IN {max_in}\n#define MBG_DEP_BATCH {dep_batch}\n#define MBG_SCAN {scan}\n"
             f"#define MBG_WSUM_REG {wsum_reg}\n#define MBG_RATE32 {rate32}\n#define MBG_MULTI {multi}\n"
             f"#define MBG_SUMW {sumw}\n#define MBG_COMPACT {compact}\n#define MBG_WSMAD {wsmad}\n"
-            f"#define MBG_SPARSE {sparse}\n"
+            f"#define MBG_SPARSE {sparse}\n#define MBG_PROG {prog}\n"
             f"#define MBG_HD __device__ __forceinline__\n{math_src}\n{kern_src}\n")
 
 
 def check_kernel(verbose=False):
     with tempfile.TemporaryDirectory() as td:
         procs = []
-        shapes = [(1, 64, 1, 1, 1, 0, 13, 4, 4, 1, 0, 0, 1, 0, 0, 0), (16, 512, 0, 0, 0, 1, 4, 4, 4, 1, 1, 0, 1, 1, 1, 0),
-                  (32, 256, 0, 0, 0, 0, 4, 4, 8, 0, 1, 1, 1, 1, 0, 0), (16, 448, 0, 1, 0, 1, 4, 8, 4, 1, 1, 0, 2, 0, 0, 0),
-                  (2, 256, 0, 1, 1, 1, 6, 2, 4, 1, 0, 0, 1, 0, 0, 1)]
-        for nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp in shapes:
-            src = os.path.join(td, f"k{nw}_{disc}_{trace}_{multi}_{cp}_{sp}.hip")
+        shapes = [(1, 64, 1, 1, 1, 0, 13, 4, 4, 1, 0, 0, 1, 0, 0, 0, 0),
+                  (16, 512, 0, 0, 0, 1, 4, 4, 4, 1, 1, 0, 1, 1, 1, 0, 0),
+                  (32, 256, 0, 0, 0, 0, 4, 4, 8, 0, 1, 1, 1, 1, 0, 0, 1),
+                  (16, 448, 0, 1, 0, 1, 4, 8, 4, 1, 1, 0, 2, 0, 0, 0, 0),
+                  (2, 256, 0, 1, 1, 1, 6, 2, 4, 1, 0, 0, 1, 0, 0, 1, 1),
+                  (200, 64, 1, 0, 1, 0, 3, 2, 4, 0, 0, 0, 1, 0, 0, 0, 1)]
+        for nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp, pg in shapes:
+            src = os.path.join(td, f"k{nw}_{disc}_{trace}_{multi}_{cp}_{sp}_{pg}.hip")
             with open(src, "w") as f:
-                f.write(kernel_source(nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp))
+                f.write(kernel_source(nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp,
+                                      pg))
             cmd = [HIPCC, *KERNEL_FLAGS, "-include", "hip/hip_runtime.h", "--cuda-device-only", "-c", src, "-o", src + ".o"]
             if verbose:
                 print("+", " ".join(cmd), flush=True)

@@ -17,6 +17,7 @@
 //   MBG_COMPACT     1: 8-byte node records and 16-bit dependents (small models, no generic rates)
 //   MBG_WSMAD       1: every rate is below 2^31 (register sum updates by multiply-add)
 //   MBG_SPARSE      1: more than 20 readouts, window cells and final states in a hash table
+//   MBG_PROG        1: some fast node reads more than 20 nodes (logic program, no truth table)
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -71,6 +72,10 @@ struct mbg_node {
 #ifndef MBG_SPARSE
 #define MBG_SPARSE 0
 #endif
+#ifndef MBG_PROG
+#define MBG_PROG 0
+#endif
+#define PROG_IN 255u  // in_cnt of a node whose logic is a program
 // Projected (readout) state: up to 20 bits in a dense table, up to 63 with MBG_SPARSE.
 #if MBG_SPARSE
 typedef unsigned long long sidx;
@@ -122,7 +127,7 @@ struct mbg_layout {
     u32 terms;     // u64[2K]: entropy terms rint(r log2(r) 2^sigma) of each readout's U and D
     u32 gen_mask;  // u64[NW]: nodes with a generic rate table
     u32 bytes;     // blob size
-    u32 pad;
+    u32 prog;      // u32[]: logic programs of nodes with in_cnt 255 ([length, ops...])
 };
 
 struct mbg_desc {
@@ -176,6 +181,38 @@ __device__ __forceinline__ u32 dep_entry(u32 v) { return (v & 0xfffu) | ((v >> 1
 #define TDEP(i) (((const u32*)(tb + D.lay.deps))[i])
 #endif
 #define TIN(i) ((u32)((const u16*)(tb + D.lay.inputs))[i])
+#define TPROG(i) (((const u32*)(tb + D.lay.prog))[i])
+
+#if MBG_PROG
+// Logic of a node that reads more than 20 nodes: a postfix program over a stack of bits held
+// in one u64 (op format in csrc/model.cpp: push node bit / constant, AND, OR, XOR, NOT,
+// SELECT), bit(j) giving the state bit of node j.  Every op runs the same branch-free
+// instructions, so lanes evaluating different nodes only differ in their loop count.  The
+// bit read is unconditional: for ops other than a push, its index (the op's low 16 bits) is
+// 0, 1 or 2, a valid node (such a model has at least 22).
+template <class Bit>
+__device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 off, Bit bit) {
+    const u32 len = TPROG(off);
+    u64 st = 0;
+    for (u32 p = 1; p <= len; p++) {
+        const u32 op = TPROG(off + p);
+        const u32 kind = op >> 30;
+        const u32 a = (u32)st & 1u, b = (u32)(st >> 1) & 1u, c = (u32)(st >> 2) & 1u;
+        const u32 lit = bit(op & 0xffffu) ^ ((op >> 16) & 1u);
+        const u32 f = op & 3u;
+        const u32 bin = f == 0u ? (a & b) : (f == 1u ? (a | b) : (a ^ b));
+        const u64 pushv = (st << 1) | (u64)(kind == 0u ? lit : (op & 1u));
+        const u64 binv = ((st >> 2) << 1) | (u64)bin;
+        const u64 selv = ((st >> 3) << 1) | (u64)(c ? b : a);
+        const u64 notv = st ^ 1ull;
+        st = (kind == 0u || kind == 3u) ? pushv : (kind == 1u ? binv : ((op & 1u) ? selv : notv));
+    }
+    return (u32)st & 1u;
+}
+#define IN_CNT(n) ((n).in_cnt == PROG_IN ? 0u : (n).in_cnt)
+#else
+#define IN_CNT(n) ((n).in_cnt)
+#endif
 #define TLUT(i) (((const u32*)(tb + D.lay.lut))[i])
 #define TRO(i) (((const u32*)(tb + D.lay.readouts))[i])
 #define TTERM(i) (((const u64*)(tb + D.lay.terms))[i])
@@ -255,6 +292,19 @@ __device__ __forceinline__ u64 rate_from_key(const mbg_desc& D, const char* tb, 
     return xi == logic ? 0ull : TRATE(2 * i + xi);
 }
 
+// The same for any node, with bit(j) reading the state for a logic program.
+template <class Bit>
+__device__ __forceinline__ u64 rate_of(const mbg_desc& D, const char* tb, const mbg_nd& n, u32 i, u32 key, Bit bit) {
+#if MBG_PROG
+    if (n.in_cnt == PROG_IN) {
+        const u32 xi = key & 1u;
+        return xi == eval_prog(D, tb, n.tab_off, bit) ? 0ull : TRATE(2 * i + xi);
+    }
+#endif
+    (void)bit;
+    return rate_from_key(D, tb, n, i, key);
+}
+
 // Initial states, active sets and word rate sums.  A block takes 64 trajectories: it draws
 // their state words into LDS (lane-interleaved) and the staging area, then each wave sweeps
 // whole words for the 64 trajectories, one per lane, so the table reads of a word's 64
@@ -286,7 +336,7 @@ __device__ __forceinline__ void init_rates_block(const mbg_desc& D, u64 t0) {
         u64 a = 0, sum = 0;
         const int nb = min(64, D.N - w * 64);
         for (int b0 = 0; b0 < nb; b0 += 8) {
-            u32 key[8], tab[8];
+            u32 key[8], tab[8], inc[8];
             u64 rup[8], rdn[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) {
@@ -297,10 +347,17 @@ __device__ __forceinline__ void init_rates_block(const mbg_desc& D, u64 t0) {
                 const mbg_nd nd = decode_compact(((const CONST_AS u64*)(tb + D.lay.nodes))[i]);
                 const u32 in_off = nd.in_off, in_cnt = nd.in_cnt;
                 tab[q] = nd.tab_off;
+                inc[q] = in_cnt;
 #else
                 const CONST_AS u32* nr = (const CONST_AS u32*)(tb + D.lay.nodes) + 4 * i;
-                const u32 in_off = nr[0], in_cnt = nr[3] >> 16 & 0xffu;
+                const u32 in_off = nr[0], in_raw = nr[3] >> 16 & 0xffu;
+#if MBG_PROG
+                const u32 in_cnt = in_raw == PROG_IN ? 0u : in_raw;
+#else
+                const u32 in_cnt = in_raw;
+#endif
                 tab[q] = nr[2];
+                inc[q] = in_raw;
 #endif
 #if !MBG_GENERIC
 #if MBG_RATE32
@@ -339,12 +396,21 @@ __device__ __forceinline__ void init_rates_block(const mbg_desc& D, u64 t0) {
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const u32 i = (u32)min(w * 64 + b0 + q, D.N - 1);
+                auto xsbit = [&](u32 j) { return (u32)(xs[(j >> 6) * INIT_TRAJ + lane] >> (j & 63)) & 1u; };
+                (void)xsbit;
 #if MBG_GENERIC
-                u64 v = rate_from_key(D, tb, TNODE(i), i, key[q]);
+                u64 v = rate_of(D, tb, TNODE(i), i, key[q], xsbit);
 #else
                 // fast form: the truth-table bit per lane, the two rates of the (uniform) node
-                const u32 bp = tab[q] + key[q];
-                const u32 logic = (TLUT(bp >> 5) >> (bp & 31)) & 1u;
+                u32 logic;
+#if MBG_PROG
+                if (inc[q] == PROG_IN) logic = eval_prog(D, tb, tab[q], xsbit);
+                else
+#endif
+                {
+                    const u32 bp = tab[q] + key[q];
+                    logic = (TLUT(bp >> 5) >> (bp & 31)) & 1u;
+                }
                 const u32 xi = key[q] & 1u;
                 const u64 v = (xi ? rdn[q] : rup[q]) & (0ull - (u64)(xi != logic));
                 (void)i;
@@ -394,11 +460,11 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
         if ((int)i >= D.N) break;
         const mbg_nd n = TNODE(i);
         u32 key = (u32)(x[i >> 6] >> (i & 63)) & 1u;
-        for (u32 m = 0; m < n.in_cnt; m++) {
+        for (u32 m = 0; m < IN_CNT(n); m++) {
             const u32 j = TIN(n.in_off + m);
             key |= ((u32)(x[j >> 6] >> (j & 63)) & 1u) << (m + 1);
         }
-        const u64 r = rate_from_key(D, tb, n, i, key);
+        const u64 r = rate_of(D, tb, n, i, key, [&](u32 j) { return (u32)(x[j >> 6] >> (j & 63)) & 1u; });
         if (r) {
             a |= 1ull << b;
             sum += r;
@@ -427,12 +493,21 @@ struct Lane {
 #pragma unroll
         for (u32 m = 0; m < MBG_MAX_IN; m++) {
             const u32 b = xbit(TIN(n.in_off + m));
-            key |= (m < n.in_cnt ? b : 0u) << (m + 1);
+            key |= (m < IN_CNT(n) ? b : 0u) << (m + 1);
         }
 #else
-        for (u32 m = 0; m < n.in_cnt; m++) key |= xbit(TIN(n.in_off + m)) << (m + 1);
+        for (u32 m = 0; m < IN_CNT(n); m++) key |= xbit(TIN(n.in_off + m)) << (m + 1);
 #endif
         return key;
+    }
+
+    // Logic value of fast node n at key (truth table), or of its program.
+    __device__ __forceinline__ u32 logic_of(const mbg_nd& n, u32 key) const {
+#if MBG_PROG
+        if (n.in_cnt == PROG_IN) return eval_prog(D, tb, n.tab_off, [&](u32 j) { return xbit(j); });
+#endif
+        const u32 bp = n.tab_off + key;
+        return (TLUT(bp >> 5) >> (bp & 31)) & 1u;
     }
 
     // Entropy term rint(r log2(r) 2^sigma) of readout ro (index) at rate r produced with
@@ -823,8 +898,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 } else
 #endif
                 {
-                    const u32 bp = nk[u].tab_off + key;
-                    const u32 logic = (TLUT(bp >> 5) >> (bp & 31)) & 1u;
+                    const u32 logic = L.logic_of(nk[u], key);
                     const u32 a_old = (u32)(LANE_A(kk[u] >> 6) >> (kk[u] & 63)) & 1u;
                     rn[u] = xn[u] != logic ? TRATE(2 * kk[u] + xn[u]) : 0ull;
                     rp[u] = a_old ? TRATE(2 * kk[u] + xo[u]) : 0ull;

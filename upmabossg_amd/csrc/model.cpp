@@ -108,6 +108,89 @@ void collect_nodes(const ParsedModel& pm, int self, const Ast* e, std::set<int>&
     }
 }
 
+// Postfix program of a fast node's logic (nodes with more than kMaxInputsFast inputs), one
+// u32 per op, evaluated on a stack of bits:
+//   bits 30-31 = 0  push the state bit of node (bits 0-15), negated when bit 16 is set
+//              = 1  pop two, push AND (bits 0-1 = 0), OR (1) or XOR (2)
+//              = 2  NOT of the top bit (bit 0 clear); SELECT (bit 0 set): pop r, m, c and
+//                   push c ? m : r
+//              = 3  push the constant bit 0
+// The rate only depends on whether the logic is non-zero (generator.cpp:93-107), and &, |,
+// ^, ! and ?: only test their operands for zero, so numbers become constant bits and unary
+// minus is the identity.  Arithmetic and comparisons are refused for such nodes.
+struct ProgCompiler {
+    const ParsedModel& pm;
+    int self;
+    std::vector<uint32_t>& out;
+    int depth = 0, max_depth = 0, alias_depth = 0;
+
+    void push_op(uint32_t op, int delta) {
+        out.push_back(op);
+        depth += delta;
+        max_depth = std::max(max_depth, depth);
+    }
+    [[noreturn]] void refuse() {
+        throw ParseError("node " + pm.nodes[self].name + " reads more than " + std::to_string(kMaxInputsFast) +
+                         " nodes: its logic may then only use & | ^ ! ?: ( ), nodes, @aliases and constants");
+    }
+    void emit(const Ast* e, bool negate = false) {
+        switch (e->kind) {
+            case Ast::Number:
+                push_op((3u << 30) | ((e->number != 0.f) != negate ? 1u : 0u), 1);
+                return;
+            case Ast::Variable:
+                push_op((3u << 30) | ((evaluate(e, pm) != 0.f) != negate ? 1u : 0u), 1);
+                return;
+            case Ast::Group:
+                emit(e->kids[0].get(), negate);
+                return;
+            case Ast::Node: {
+                const int i = pm.index_of(e->text);
+                if (i < 0) throw ParseError("unknown node name: " + e->text);
+                push_op((uint32_t)i | (negate ? 1u << 16 : 0u), 1);
+                return;
+            }
+            case Ast::Alias: {
+                if (++alias_depth > 64) throw ParseError("alias recursion too deep");
+                const Ast* a = pm.nodes[self].attr(e->text.substr(1));
+                if (!a) throw ParseError("Node " + pm.nodes[self].name + " does not have attribute " + e->text.substr(1));
+                emit(a, negate);
+                alias_depth--;
+                return;
+            }
+            case Ast::Unary:
+                if (e->oper == Ast::Not) {
+                    emit(e->kids[0].get(), !negate);
+                    return;
+                }
+                if (e->oper == Ast::Sub || e->oper == Ast::Add || e->oper == Ast::Pos) {
+                    emit(e->kids[0].get(), negate);
+                    return;
+                }
+                refuse();
+            case Ast::Ternary:
+                emit(e->kids[0].get());
+                emit(e->kids[1].get(), negate);
+                emit(e->kids[2].get(), negate);
+                push_op((2u << 30) | 1u, -2);
+                return;
+            case Ast::Binary: {
+                uint32_t code;
+                if (e->oper == Ast::And) code = 0;
+                else if (e->oper == Ast::Or) code = 1;
+                else if (e->oper == Ast::Xor) code = 2;
+                else refuse();
+                emit(e->kids[0].get());
+                emit(e->kids[1].get());
+                push_op((1u << 30) | code, -1);
+                if (negate) push_op(2u << 30, 0);
+                return;
+            }
+        }
+        refuse();
+    }
+};
+
 void check_rate(float v, const std::string& node) {
     if (!(v >= 0.f) || std::isinf(v))
         throw ParseError("node " + node + ": rates must be finite and non-negative");
@@ -181,16 +264,18 @@ CompiledModel compile_model(const ParsedModel& pm) {
         }
         const bool self_ref = refs.erase(i) > 0;
         const int cnt = (int)refs.size();
-        cm.max_inputs = std::max(cm.max_inputs, cnt);
-        if (cnt > (fast ? kMaxInputsFast : kMaxInputsGeneric))
+        const bool prog = fast && cnt > kMaxInputsFast;  // logic program instead of a truth table
+        if (!fast && cnt > kMaxInputsGeneric)
             throw ParseError("node " + nd.name + " reads " + std::to_string(cnt) + " nodes (limit " +
-                             std::to_string(fast ? kMaxInputsFast : kMaxInputsGeneric) + ")");
+                             std::to_string(kMaxInputsGeneric) + " for rates other than @logic ? U : 0)");
+        if (!prog) cm.max_inputs = std::max(cm.max_inputs, cnt);
         rec.in_off = (uint32_t)cm.inputs.size();
-        rec.in_cnt = (uint8_t)cnt;
+        rec.in_cnt = prog ? kProgInputs : (uint8_t)cnt;
         std::vector<int> in(refs.begin(), refs.end());
         for (int m = 0; m < cnt; m++) {
             cm.inputs.push_back((uint32_t)in[m]);
-            dep_lists[in[m]].push_back((uint32_t)i | ((uint32_t)(m + 1) << 24));
+            // key bit position of the input (a program node has no key: any non-zero position)
+            dep_lists[in[m]].push_back((uint32_t)i | ((uint32_t)(prog ? 1 : m + 1) << 24));
         }
         // A fast node whose logic does not read itself has rate 0 right after it flips
         // (it was active: x_i != logic; flipping makes them equal), so the kernel updates it
@@ -199,8 +284,24 @@ CompiledModel compile_model(const ParsedModel& pm) {
             dep_lists[i].push_back((uint32_t)i);
             rec.flags |= 0x40;
         }
-        const uint32_t nkeys = 1u << (cnt + 1);
         StateEval ev{pm, i, bits};
+        if (prog) {
+            check_rate(u, nd.name);
+            check_rate(d, nd.name);
+            cm.up[i] = u;
+            cm.down[i] = d;
+            cm.any_prog = true;
+            rec.tab_off = (uint32_t)cm.prog.size();
+            cm.prog.push_back(0u);  // length, filled below
+            ProgCompiler pc{pm, i, cm.prog};
+            pc.emit(logic);
+            if (pc.max_depth > kMaxProgDepth)
+                throw ParseError("node " + nd.name + ": logic nests too deeply (bit stack above " +
+                                 std::to_string(kMaxProgDepth) + ")");
+            cm.prog[rec.tab_off] = (uint32_t)(cm.prog.size() - rec.tab_off - 1);
+            continue;
+        }
+        const uint32_t nkeys = 1u << (cnt + 1);
         if (fast) {
             check_rate(u, nd.name);
             check_rate(d, nd.name);

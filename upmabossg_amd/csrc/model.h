@@ -6,7 +6,9 @@
 // Here each node becomes a truth table over its key = x_i | (input bits << 1):
 //   fast path  one logic bit per key, plus the FP32 constants U_i, D_i
 //   generic    one FP32 rate per key
-// so that every lane of a wave evaluates any node with the same instructions.
+// so that every lane of a wave evaluates any node with the same instructions.  A fast node
+// that reads more than kMaxInputsFast nodes (the ring models of data/generate-synth.py read
+// 2 sqrt(N) + 2) gets a postfix logic program instead, run by the same loop on every lane.
 #pragma once
 
 #include <cstdint>
@@ -28,7 +30,9 @@ struct NodeRec {
 };
 static_assert(sizeof(NodeRec) == 16, "NodeRec must match the device layout");
 
-constexpr int kMaxInputsFast = 20;     // LUT of 2^21 bits
+constexpr int kMaxInputsFast = 20;     // LUT of 2^21 bits; wider fast nodes run a logic program
+constexpr uint8_t kProgInputs = 255;   // NodeRec::in_cnt of a node whose logic is a program
+constexpr int kMaxProgDepth = 64;      // bit-stack depth of a logic program
 constexpr int kMaxInputsGeneric = 16;  // table of 2^17 rates
 constexpr int kMaxReadouts = 20;       // dense window tables (the reference's limit, src/main.cpp:142-146)
 constexpr int kMaxSparseReadouts = 63; // sparse window tables (hash of window and projected state)
@@ -45,13 +49,17 @@ struct CompiledModel {
 
     std::vector<NodeRec> nodes;
     std::vector<uint32_t> inputs, deps, lut;
+    // logic programs of fast nodes with more than kMaxInputsFast inputs (in_cnt = kProgInputs,
+    // tab_off = offset of [length, ops...]); op format in model.cpp
+    std::vector<uint32_t> prog;
+    bool any_prog = false;
     std::vector<uint8_t> generic;
     std::vector<uint64_t> gen_mask;   // per 64-node word
     std::vector<float> up, down;      // fast nodes, FP32
     std::vector<float> gtab;          // generic nodes: FP32 rate per key, at NodeRec::tab_off
     std::vector<float> istate;
     bool any_generic = false;
-    int max_inputs = 0;               // largest in_cnt
+    int max_inputs = 0;               // largest in_cnt of a truth-table node
 };
 
 // Entropy term of a fixed-point rate: rint(r * log2(r) * 2^sigma), 0 for r = 0.

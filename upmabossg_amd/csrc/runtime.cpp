@@ -71,7 +71,7 @@ void copy_err(char* err, size_t n, const std::string& s) {
 // Byte offsets of the model tables in one blob; identical to struct mbg_layout in
 // device/mbg_kernel.hip.
 struct Layout {
-    uint32_t nodes, rates, deps, inputs, lut, readouts, terms, gen_mask, bytes, pad;
+    uint32_t nodes, rates, deps, inputs, lut, readouts, terms, gen_mask, bytes, prog;
 };
 
 // Kernel argument block; layout identical to struct mbg_desc in device/mbg_kernel.hip.
@@ -104,12 +104,13 @@ using mbg_desc_ptr = const void*;
 
 struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 2, scan = 4, wsum_reg = 0, rate32 = 0,
-        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0;
+        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
-                        sumw, compact, wsmad, sparse) < std::tie(o.nw, o.block, o.discrete, o.trace, o.generic,
-                                                                 o.tables_lds, o.max_in, o.dep_batch, o.scan, o.wsum_reg,
-                                                                 o.rate32, o.multi, o.sumw, o.compact, o.wsmad, o.sparse);
+                        sumw, compact, wsmad, sparse, prog) < std::tie(o.nw, o.block, o.discrete, o.trace, o.generic,
+                                                                       o.tables_lds, o.max_in, o.dep_batch, o.scan,
+                                                                       o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact,
+                                                                       o.wsmad, o.sparse, o.prog);
     }
 };
 
@@ -133,6 +134,7 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_COMPACT " << sh.compact << "\n";
     os << "#define MBG_WSMAD " << sh.wsmad << "\n";
     os << "#define MBG_SPARSE " << sh.sparse << "\n";
+    os << "#define MBG_PROG " << sh.prog << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -163,7 +165,7 @@ bool rates_fit_i31(const mbg::RateTables& rt) {
 // The compact table layout (8-byte node records, 16-bit dependents) fits when the model has
 // no generic rate tables and its tables stay within the packed field widths.
 bool compact_fits(const mbg::CompiledModel& cm) {
-    if (cm.any_generic || cm.N > 4096 || cm.K > 31 || cm.max_inputs > 15) return false;
+    if (cm.any_generic || cm.any_prog || cm.N > 4096 || cm.K > 31 || cm.max_inputs > 15) return false;
     if (cm.inputs.size() + 32 >= (1u << 14) || cm.deps.size() >= (1u << 14)) return false;
     for (const mbg::NodeRec& r : cm.nodes)
         if (r.dep_cnt > 255 || (uint64_t)r.tab_off + (2ull << r.in_cnt) > (1ull << 16)) return false;
@@ -216,6 +218,7 @@ Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt, bool rat
     b.lay.readouts = seg(ro.data(), ro.size() * 4);
     b.lay.terms = seg(rt.terms.data(), rt.terms.size() * 8);
     b.lay.gen_mask = seg(cm.gen_mask.data(), cm.gen_mask.size() * 8);
+    b.lay.prog = seg(cm.prog.data(), cm.prog.size() * 4);
     bytes.resize((bytes.size() + 15) & ~(size_t)15);
     b.lay.bytes = (uint32_t)bytes.size();
     b.words.resize(bytes.size() / 8);
@@ -697,6 +700,7 @@ Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::R
     sh.compact = compact ? 1 : 0;
     sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
     sh.sparse = cm.K > mbg::kMaxReadouts ? 1 : 0;
+    sh.prog = cm.any_prog ? 1 : 0;
     return sh;
 }
 
@@ -725,6 +729,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
         sh = (!p->block && tu != c->tuned.end()) ? tu->second : default_shape(c, cm, rt, p->block, p->trace);
     }
     sh.sparse = sparse ? 1 : 0;
+    sh.prog = cm.any_prog ? 1 : 0;
     const Blob blob = pack_blob(cm, rt, sh.rate32 != 0, sh.compact != 0);
     if ((size_t)lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0) >
         (size_t)c->lds_per_block)
@@ -1088,6 +1093,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         SweepClass& C = *sw->classes[it->second];
         C.members.push_back(v);
         C.sh.generic |= cm.any_generic ? 1 : 0;
+        C.sh.prog |= cm.any_prog ? 1 : 0;
         C.sh.max_in = std::max(C.sh.max_in, cm.max_inputs);
         C.sh.rate32 &= fits32[v];
         C.sh.compact &= compact_fits(cm) ? 1 : 0;
@@ -1614,8 +1620,10 @@ mbg_status mbg_model_get_variant(const mbg_model* m, float* mult_up, float* mult
 mbg_status mbg_model_generate_source(const mbg_model* m, int trace, char* buf, size_t cap, size_t* len) {
     if (!m) return MBG_ERR_ARG;
     const mbg::CompiledModel& cm = m->cm;
-    const std::string s = generate_source(cm, Shape{cm.NW, 64, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, 0,
-                                                    std::max(1, cm.max_inputs)});
+    Shape sh{cm.NW, 64, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, 0, std::max(1, cm.max_inputs)};
+    sh.sparse = cm.K > mbg::kMaxReadouts ? 1 : 0;
+    sh.prog = cm.any_prog ? 1 : 0;
+    const std::string s = generate_source(cm, sh);
     if (len) *len = s.size();
     if (buf && cap) {
         const size_t k = std::min(cap - 1, s.size());

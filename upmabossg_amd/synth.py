@@ -161,3 +161,49 @@ def sweep_variants(count=256, seed=1, n_lo=50, n_hi=2000, traj_lo=1000, traj_hi=
             net.istate[i] = 0.0
         out.append(net)
     return out
+
+
+def ring_network(nodes=100, noninternals=5, signal_length=0, steps_to_fixed_point=0, sample_count=1000000,
+                 time_tick=5.0, max_time=100.0, discrete=False, contiguous_noninternals=False):
+    """The ring models of the reference's benchmark generator (data/generate-synth.py:5-100),
+    as (bnd text, cfg text): a signal of s = signal_length (default int(sqrt(nodes))) ON nodes
+    travels around a ring.  Node i turns ON when it is OFF, its s predecessors are ON and the
+    node before them is OFF, and turns OFF when it and its s successors are ON, so its logic
+    reads 2s + 1 other nodes (22 at 100 nodes, 64 at 1,000).  With steps_to_fixed_point > 0 a
+    last node NFP (rate_up 1/steps, rate_down 0) switches every ring node ON for good, which
+    makes all-ON a fixed point.  Unit rates; the first s nodes start ON; every
+    (nodes // noninternals)-th node (or the first noninternals nodes when contiguous) and NFP
+    are non-internal."""
+    fixed = steps_to_fixed_point != 0
+    ring = nodes - 1 if fixed else nodes
+    s = signal_length if signal_length > 0 else int(np.sqrt(nodes))
+    stride = 1 if contiguous_noninternals else nodes // noninternals
+
+    def name(k):
+        return f"N{k % ring}"
+
+    bnd = []
+    for i in range(ring):
+        on_terms = [f"!N{i}"] + (["!NFP"] if fixed else []) + [name(i - j + ring) for j in range(1, s + 1)]
+        on_terms.append("!" + name(i - s - 1 + ring))
+        stay = " & ".join([f"N{i}"] + [name(i + j) for j in range(1, s + 1)])
+        logic = f"({' & '.join(on_terms)}) | (!({stay}) & N{i}" + (" & !NFP" if fixed else "") + ")"
+        if fixed:
+            logic += " | (NFP)"
+        bnd.append(f"Node N{i} {{\n    logic = {logic};\n    rate_up = @logic ? $u_N{i} : 0;\n"
+                   f"    rate_down = @logic ? 0 : $d_N{i};\n}}\n")
+    if fixed:
+        bnd.append("Node NFP {\n    logic = !NFP;\n    rate_up = @logic ? $u_NFP : 0;\n    rate_down = @logic ? 0 : $d_NFP;\n}\n")
+    cfg = [f"sample_count = {sample_count};", f"time_tick = {time_tick:g};", f"max_time = {max_time:g};",
+           f"discrete_time = {int(discrete)};"]
+    if fixed:
+        cfg += [f"$u_NFP = {1.0 / steps_to_fixed_point!r};", "$d_NFP = 0;", "NFP.istate = 0;", "NFP.is_internal = 0;"]
+    left = noninternals
+    for i in range(ring):
+        cfg += [f"$u_N{i} = 1;", f"$d_N{i} = 1;", f"N{i}.istate = {1 if i < s else 0};"]
+        if i % stride == 0 and left > 0:
+            cfg.append(f"N{i}.is_internal = 0;")
+            left -= 1
+        else:
+            cfg.append(f"N{i}.is_internal = 1;")
+    return "\n".join(bnd), "\n".join(cfg) + "\n"
