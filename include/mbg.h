@@ -67,6 +67,12 @@ mbg_status mbg_model_set_variant(mbg_model* m, const float* mult_up, const float
  * (the model's own where none is set), N values each; any pointer may be NULL. */
 mbg_status mbg_model_get_variant(const mbg_model* m, float* mult_up, float* mult_down, float* istate);
 
+/* The FP32 transition rate of every node at a state (state_words = ceil(N/64) words, node i
+ * = bit i % 64 of word i / 64), from the tables the kernel reads (truth tables, logic
+ * programs, generic rate tables), without variant multipliers: the host form of the
+ * generated compute_transition_rates (src/generator.cpp:124-144). */
+mbg_status mbg_model_transition_rates(const mbg_model* m, const uint64_t* state, float* rates);
+
 /* The hipRTC source generated for this model (replaces generator::generate_code,
  * src/generator.cpp:11-55).  Writes at most cap bytes including the NUL; *len gets the full
  * length.  trace != 0 adds per-trajectory summaries. */

@@ -121,6 +121,39 @@ def test_wide_fast_nodes_get_logic_programs():
     assert "limit 16" in str(e.value)
 
 
+@pytest.mark.parametrize("case", ["ring100", "ring1000", "wide1", "wide2", "wide3", "small", "generic"])
+def test_compiled_rates_equal_the_oracle_on_random_states(case):
+    """The kernel's tables (truth tables, logic programs with clauses, generic rate tables),
+    evaluated on the host, give the oracle's FP32 rate for every node at random states: this
+    pins the logic compiler on the CPU (the oracle evaluates the expression trees directly)."""
+    from tests.wide_models import wide_model
+
+    if case.startswith("ring"):
+        b, c = synth.ring_network(int(case[4:]), steps_to_fixed_point=100, sample_count=100)
+    elif case.startswith("wide"):
+        b, c = wide_model(seed=int(case[4:]))
+    elif case == "small":
+        net = synth.config_small()
+        b, c = net.bnd(), net.cfg()
+    else:
+        b, c = EDGE_GENERIC
+    m, o = Model(b, c), OracleModel(b, c)
+    rng = np.random.default_rng(7)
+    N = m.num_nodes
+    for density in [0.1, 0.5, 0.9]:
+        for _ in range(20 if N <= 100 else 3):
+            bits = rng.random(N) < density
+            words = o._words(bits)
+            got = m.transition_rates(words)
+            want = np.array([o.rate_f32(i, bits) for i in range(N)], np.float32)
+            assert np.array_equal(got, want)
+
+
+EDGE_GENERIC = ("node A { logic = B; rate_up = B ? 2.5 : (C ? 0.5 : 0); rate_down = A & !C ? 1 : 3; }\n"
+                "node B { logic = A ^ C; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n"
+                "node C { logic = !A | B; rate_up = @logic ? 1.5 : 0; rate_down = @logic ? 0 : 0.25; }\n", "")
+
+
 def test_grammar_details():
     # keywords are case-insensitive, TRUE/FALSE, comments, precedence of ?: and unary minus
     bnd = ("Node A { logic = B and NOT C Or false; rate_up = @logic ? 2*3-4/2 : 0; rate_down = @logic ? 0 : -(-1.5e0); }\n"

@@ -113,6 +113,7 @@ def lib():
         "mbg_model_constant": (C.c_int, [vp, cp, f64p]),
         "mbg_model_set_variant": (C.c_int, [vp, f32p, f32p, f32p]),
         "mbg_model_get_variant": (C.c_int, [vp, f32p, f32p, f32p]),
+        "mbg_model_transition_rates": (C.c_int, [vp, u64p, f32p]),
         "mbg_model_generate_source": (C.c_int, [vp, C.c_int, cp, sz, C.POINTER(sz)]),
         "mbg_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
         "mbg_destroy": (None, [vp]),
@@ -238,6 +239,15 @@ class Model:
         st = lib().mbg_model_set_variant(self._h, ptr(mult_up), ptr(mult_down), ptr(istate))
         if st != MBG_OK:
             raise MbgError(st, "invalid variant parameters")
+
+    def transition_rates(self, state_words) -> np.ndarray:
+        """FP32 rate of every node at a state given as ceil(N/64) uint64 words."""
+        w = np.ascontiguousarray(state_words, dtype=np.uint64)
+        out = np.zeros(self.num_nodes, np.float32)
+        st = lib().mbg_model_transition_rates(self._h, _u64p(w), out.ctypes.data_as(C.POINTER(C.c_float)))
+        if st != MBG_OK:
+            raise MbgError(st, "transition_rates failed")
+        return out
 
     def get_variant(self):
         """(mult_up, mult_down, istate) in force, one value per node."""

@@ -185,27 +185,36 @@ __device__ __forceinline__ u32 dep_entry(u32 v) { return (v & 0xfffu) | ((v >> 1
 
 #if MBG_PROG
 // Logic of a node that reads more than 20 nodes: a postfix program over a stack of bits held
-// in one u64 (op format in csrc/model.cpp: push node bit / constant, AND, OR, XOR, NOT,
-// SELECT), bit(j) giving the state bit of node j.  Every op runs the same branch-free
-// instructions, so lanes evaluating different nodes only differ in their loop count.  The
-// bit read is unconditional: for ops other than a push, its index (the op's low 16 bits) is
-// 0, 1 or 2, a valid node (such a model has at least 22).
-template <class Bit>
-__device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 off, Bit bit) {
-    const u32 len = TPROG(off);
+// in one u64 (op format in csrc/model.cpp: push node bit / constant / clause, AND, OR, XOR,
+// NOT, SELECT), word(w) giving state word w.  Every op runs the same branch-free
+// instructions, so lanes evaluating different nodes only differ in their loop count.  Each op
+// reads one state word: the clause's, or the pushed node's (for other ops the index is node
+// 0, 1 or 2, always valid).  The 5 words after an op are read unconditionally (the table is
+// padded past its last op).
+template <class Word>
+__device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 off, Word word) {
+    const u32 end = off + 1u + TPROG(off);
     u64 st = 0;
-    for (u32 p = 1; p <= len; p++) {
-        const u32 op = TPROG(off + p);
+    for (u32 p = off + 1u; p < end;) {
+        const u32 op = TPROG(p);
         const u32 kind = op >> 30;
+        const bool clause = kind == 3u && ((op >> 29) & 1u);
+        const u32 cw = TPROG(p + 1u);
+        const u64 cm = (u64)TPROG(p + 2u) | ((u64)TPROG(p + 3u) << 32);
+        const u64 cv = (u64)TPROG(p + 4u) | ((u64)TPROG(p + 5u) << 32);
+        const u32 j = op & 0xffffu;
+        const u64 wv = word(clause ? cw : (j >> 6));
         const u32 a = (u32)st & 1u, b = (u32)(st >> 1) & 1u, c = (u32)(st >> 2) & 1u;
-        const u32 lit = bit(op & 0xffffu) ^ ((op >> 16) & 1u);
+        const u32 lit = clause ? ((u32)((wv & cm) == cv) ^ ((op >> 28) & 1u))
+                               : (kind == 0u ? (((u32)(wv >> (j & 63)) & 1u) ^ ((op >> 16) & 1u)) : (op & 1u));
         const u32 f = op & 3u;
         const u32 bin = f == 0u ? (a & b) : (f == 1u ? (a | b) : (a ^ b));
-        const u64 pushv = (st << 1) | (u64)(kind == 0u ? lit : (op & 1u));
+        const u64 pushv = (st << 1) | (u64)lit;
         const u64 binv = ((st >> 2) << 1) | (u64)bin;
         const u64 selv = ((st >> 3) << 1) | (u64)(c ? b : a);
         const u64 notv = st ^ 1ull;
         st = (kind == 0u || kind == 3u) ? pushv : (kind == 1u ? binv : ((op & 1u) ? selv : notv));
+        p += clause ? 6u : 1u;
     }
     return (u32)st & 1u;
 }
@@ -292,16 +301,16 @@ __device__ __forceinline__ u64 rate_from_key(const mbg_desc& D, const char* tb, 
     return xi == logic ? 0ull : TRATE(2 * i + xi);
 }
 
-// The same for any node, with bit(j) reading the state for a logic program.
-template <class Bit>
-__device__ __forceinline__ u64 rate_of(const mbg_desc& D, const char* tb, const mbg_nd& n, u32 i, u32 key, Bit bit) {
+// The same for any node, with word(w) reading the state for a logic program.
+template <class Word>
+__device__ __forceinline__ u64 rate_of(const mbg_desc& D, const char* tb, const mbg_nd& n, u32 i, u32 key, Word word) {
 #if MBG_PROG
     if (n.in_cnt == PROG_IN) {
         const u32 xi = key & 1u;
-        return xi == eval_prog(D, tb, n.tab_off, bit) ? 0ull : TRATE(2 * i + xi);
+        return xi == eval_prog(D, tb, n.tab_off, word) ? 0ull : TRATE(2 * i + xi);
     }
 #endif
-    (void)bit;
+    (void)word;
     return rate_from_key(D, tb, n, i, key);
 }
 
@@ -396,15 +405,15 @@ __device__ __forceinline__ void init_rates_block(const mbg_desc& D, u64 t0) {
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const u32 i = (u32)min(w * 64 + b0 + q, D.N - 1);
-                auto xsbit = [&](u32 j) { return (u32)(xs[(j >> 6) * INIT_TRAJ + lane] >> (j & 63)) & 1u; };
-                (void)xsbit;
+                auto xsword = [&](u32 w) { return xs[w * INIT_TRAJ + lane]; };
+                (void)xsword;
 #if MBG_GENERIC
-                u64 v = rate_of(D, tb, TNODE(i), i, key[q], xsbit);
+                u64 v = rate_of(D, tb, TNODE(i), i, key[q], xsword);
 #else
                 // fast form: the truth-table bit per lane, the two rates of the (uniform) node
                 u32 logic;
 #if MBG_PROG
-                if (inc[q] == PROG_IN) logic = eval_prog(D, tb, tab[q], xsbit);
+                if (inc[q] == PROG_IN) logic = eval_prog(D, tb, tab[q], xsword);
                 else
 #endif
                 {
@@ -464,7 +473,7 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
             const u32 j = TIN(n.in_off + m);
             key |= ((u32)(x[j >> 6] >> (j & 63)) & 1u) << (m + 1);
         }
-        const u64 r = rate_of(D, tb, n, i, key, [&](u32 j) { return (u32)(x[j >> 6] >> (j & 63)) & 1u; });
+        const u64 r = rate_of(D, tb, n, i, key, [&](u32 w) { return x[w]; });
         if (r) {
             a |= 1ull << b;
             sum += r;
@@ -504,7 +513,7 @@ struct Lane {
     // Logic value of fast node n at key (truth table), or of its program.
     __device__ __forceinline__ u32 logic_of(const mbg_nd& n, u32 key) const {
 #if MBG_PROG
-        if (n.in_cnt == PROG_IN) return eval_prog(D, tb, n.tab_off, [&](u32 j) { return xbit(j); });
+        if (n.in_cnt == PROG_IN) return eval_prog(D, tb, n.tab_off, [&](u32 w) { return LANE_X(w); });
 #endif
         const u32 bp = n.tab_off + key;
         return (TLUT(bp >> 5) >> (bp & 31)) & 1u;

@@ -1,5 +1,7 @@
 #include "model.h"
 
+#include <map>
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -108,13 +110,18 @@ void collect_nodes(const ParsedModel& pm, int self, const Ast* e, std::set<int>&
     }
 }
 
-// Postfix program of a fast node's logic (nodes with more than kMaxInputsFast inputs), one
-// u32 per op, evaluated on a stack of bits:
+// Postfix program of a fast node's logic (nodes with more than kMaxInputsFast inputs),
+// evaluated on a stack of bits.  An op is one u32:
 //   bits 30-31 = 0  push the state bit of node (bits 0-15), negated when bit 16 is set
 //              = 1  pop two, push AND (bits 0-1 = 0), OR (1) or XOR (2)
 //              = 2  NOT of the top bit (bit 0 clear); SELECT (bit 0 set): pop r, m, c and
 //                   push c ? m : r
-//              = 3  push the constant bit 0
+//              = 3  push the constant bit 0 (bit 29 clear), or a clause (bit 29 set): push
+//                   (x[w] & mask) == value, negated when bit 28 is set, with w, mask (lo, hi)
+//                   and value (lo, hi) in the 5 words after the op
+// Operands of an AND (OR) chain that are literals are grouped by state word into clauses:
+// x[w] & mask == value tests every literal of that word at once (for an OR, the negated
+// clause of the negated literals), so a chain of 60 literals over 2 words costs 2 clauses.
 // The rate only depends on whether the logic is non-zero (generator.cpp:93-107), and &, |,
 // ^, ! and ?: only test their operands for zero, so numbers become constant bits and unary
 // minus is the identity.  Arithmetic and comparisons are refused for such nodes.
@@ -132,6 +139,98 @@ struct ProgCompiler {
     [[noreturn]] void refuse() {
         throw ParseError("node " + pm.nodes[self].name + " reads more than " + std::to_string(kMaxInputsFast) +
                          " nodes: its logic may then only use & | ^ ! ?: ( ), nodes, @aliases and constants");
+    }
+    // e through parentheses, unary plus / minus and @aliases
+    const Ast* strip(const Ast* e) {
+        for (int hops = 0;; hops++) {
+            if (hops > 64) throw ParseError("alias recursion too deep");
+            if (e->kind == Ast::Group || (e->kind == Ast::Unary && (e->oper == Ast::Sub || e->oper == Ast::Pos))) {
+                e = e->kids[0].get();
+            } else if (e->kind == Ast::Alias) {
+                const Ast* a = pm.nodes[self].attr(e->text.substr(1));
+                if (!a) throw ParseError("Node " + pm.nodes[self].name + " does not have attribute " + e->text.substr(1));
+                e = a;
+            } else {
+                return e;
+            }
+        }
+    }
+    // [!]...[!]node: its index and whether it is taken positively
+    bool literal(const Ast* e, int& node, bool& positive) {
+        positive = true;
+        for (int hops = 0; hops < 256; hops++) {
+            e = strip(e);
+            if (e->kind == Ast::Unary && e->oper == Ast::Not) {
+                positive = !positive;
+                e = e->kids[0].get();
+                continue;
+            }
+            if (e->kind != Ast::Node) return false;
+            node = pm.index_of(e->text);
+            if (node < 0) throw ParseError("unknown node name: " + e->text);
+            return true;
+        }
+        return false;
+    }
+    void flatten(const Ast* e, Ast::Oper op, std::vector<const Ast*>& out) {
+        const Ast* s = strip(e);
+        if (s->kind == Ast::Binary && s->oper == op) {
+            flatten(s->kids[0].get(), op, out);
+            flatten(s->kids[1].get(), op, out);
+        } else {
+            out.push_back(e);
+        }
+    }
+    void push_clause(uint32_t w, uint64_t mask, uint64_t value, bool neg) {
+        push_op((3u << 30) | (1u << 29) | (neg ? 1u << 28 : 0u), 1);
+        out.push_back(w);
+        out.push_back((uint32_t)mask);
+        out.push_back((uint32_t)(mask >> 32));
+        out.push_back((uint32_t)value);
+        out.push_back((uint32_t)(value >> 32));
+    }
+    // AND / OR chain: literals grouped into one clause per state word, other operands
+    // compiled on their own, combined left to right (stack depth stays at 2 above theirs)
+    void emit_chain(const Ast* e, Ast::Oper op, bool negate) {
+        std::vector<const Ast*> ops;
+        flatten(e, op, ops);
+        const bool is_or = op == Ast::Or;
+        std::map<uint32_t, std::pair<uint64_t, uint64_t>> words;  // word -> (mask, required value)
+        std::vector<const Ast*> rest;
+        for (const Ast* o : ops) {
+            int j;
+            bool pos;
+            if (!literal(o, j, pos)) {
+                rest.push_back(o);
+                continue;
+            }
+            // AND: every literal true (x_j == pos); OR: not every literal false (x_j == !pos)
+            const bool want = is_or ? !pos : pos;
+            const uint64_t bit = 1ull << (j & 63);
+            auto& mv = words[(uint32_t)j >> 6];
+            if (mv.first & bit) {
+                if (((mv.second & bit) != 0) != want) {  // A & !A is false, A | !A is true
+                    push_op((3u << 30) | ((is_or != negate) ? 1u : 0u), 1);
+                    return;
+                }
+            } else {
+                mv.first |= bit;
+                if (want) mv.second |= bit;
+            }
+        }
+        const uint32_t code = is_or ? 1u : 0u;
+        bool first = true;
+        for (const auto& kv : words) {
+            push_clause(kv.first, kv.second.first, kv.second.second, is_or);
+            if (!first) push_op((1u << 30) | code, -1);
+            first = false;
+        }
+        for (const Ast* r : rest) {
+            emit(r);
+            if (!first) push_op((1u << 30) | code, -1);
+            first = false;
+        }
+        if (negate) push_op(2u << 30, 0);
     }
     void emit(const Ast* e, bool negate = false) {
         switch (e->kind) {
@@ -175,10 +274,12 @@ struct ProgCompiler {
                 push_op((2u << 30) | 1u, -2);
                 return;
             case Ast::Binary: {
+                if (e->oper == Ast::And || e->oper == Ast::Or) {
+                    emit_chain(e, e->oper, negate);
+                    return;
+                }
                 uint32_t code;
-                if (e->oper == Ast::And) code = 0;
-                else if (e->oper == Ast::Or) code = 1;
-                else if (e->oper == Ast::Xor) code = 2;
+                if (e->oper == Ast::Xor) code = 2;
                 else refuse();
                 emit(e->kids[0].get());
                 emit(e->kids[1].get());
@@ -197,6 +298,59 @@ void check_rate(float v, const std::string& node) {
 }
 
 }  // namespace
+
+namespace {
+
+// Host evaluation of a logic program (the op format above), as the kernel runs it.
+uint32_t run_program(const CompiledModel& cm, uint32_t off, const uint64_t* x) {
+    const uint32_t end = off + 1 + cm.prog[off];
+    uint64_t st = 0;
+    for (uint32_t p = off + 1; p < end;) {
+        const uint32_t op = cm.prog[p];
+        const uint32_t kind = op >> 30;
+        const uint64_t a = st & 1u, b = (st >> 1) & 1u, c = (st >> 2) & 1u;
+        if (kind == 3 && ((op >> 29) & 1u)) {
+            const uint64_t m = (uint64_t)cm.prog[p + 2] | (uint64_t)cm.prog[p + 3] << 32;
+            const uint64_t v = (uint64_t)cm.prog[p + 4] | (uint64_t)cm.prog[p + 5] << 32;
+            st = (st << 1) | (uint64_t)(((x[cm.prog[p + 1]] & m) == v) != (((op >> 28) & 1u) != 0));
+            p += 6;
+            continue;
+        }
+        if (kind == 0) {
+            const uint32_t j = op & 0xffffu;
+            st = (st << 1) | (((x[j >> 6] >> (j & 63)) & 1u) ^ ((op >> 16) & 1u));
+        } else if (kind == 1) {
+            const uint32_t f = op & 3u;
+            st = ((st >> 2) << 1) | (f == 0 ? (a & b) : f == 1 ? (a | b) : (a ^ b));
+        } else if (kind == 2) {
+            st = (op & 1u) ? (((st >> 3) << 1) | (c ? b : a)) : (st ^ 1u);
+        } else {
+            st = (st << 1) | (op & 1u);
+        }
+        p++;
+    }
+    return (uint32_t)(st & 1u);
+}
+
+}  // namespace
+
+float node_rate(const CompiledModel& cm, int i, const uint64_t* x) {
+    const NodeRec& r = cm.nodes[i];
+    const uint32_t xi = (uint32_t)(x[i >> 6] >> (i & 63)) & 1u;
+    if (r.in_cnt == kProgInputs) {
+        const uint32_t logic = run_program(cm, r.tab_off, x);
+        return xi == logic ? 0.f : (xi ? cm.down[i] : cm.up[i]);
+    }
+    uint32_t key = xi;
+    for (int m = 0; m < r.in_cnt; m++) {
+        const uint32_t j = cm.inputs[r.in_off + m];
+        key |= (uint32_t)((x[j >> 6] >> (j & 63)) & 1u) << (m + 1);
+    }
+    if (cm.generic[i]) return cm.gtab[r.tab_off + key];
+    const uint64_t bp = (uint64_t)r.tab_off + key;
+    const uint32_t logic = (cm.lut[bp >> 5] >> (bp & 31)) & 1u;
+    return xi == logic ? 0.f : (xi ? cm.down[i] : cm.up[i]);
+}
 
 uint64_t entropy_term(int sigma, uint64_t r) {
     if (!r) return 0;
@@ -334,6 +488,7 @@ CompiledModel compile_model(const ParsedModel& pm) {
         bits[i] = 0;
         for (int m = 0; m < cnt; m++) bits[in[m]] = 0;
     }
+    if (cm.any_prog) cm.prog.resize(cm.prog.size() + 8, 0u);  // the evaluator reads 5 words past an op
     for (int i = 0; i < cm.N; i++) {
         if (dep_lists[i].size() > 0xffff) throw ParseError("node " + cm.names[i] + " has too many dependents");
         cm.nodes[i].dep_off = (uint32_t)cm.deps.size();

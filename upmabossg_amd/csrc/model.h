@@ -82,6 +82,10 @@ CompiledModel compile_model(const ParsedModel& pm);
 RateTables make_rate_tables(const CompiledModel& cm, const float* mult_up, const float* mult_down,
                             const float* istate);
 
+// FP32 rate of node i at a state (NW words), from the compiled tables: the truth table or
+// logic program of a fast node with U_i / D_i, or the generic rate table (no multipliers).
+float node_rate(const CompiledModel& cm, int i, const uint64_t* state);
+
 int ceil_log2_u64(uint64_t v);
 int ceil_log2_real(double q);
 

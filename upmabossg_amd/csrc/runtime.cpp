@@ -1617,6 +1617,12 @@ mbg_status mbg_model_get_variant(const mbg_model* m, float* mult_up, float* mult
     return MBG_OK;
 }
 
+mbg_status mbg_model_transition_rates(const mbg_model* m, const uint64_t* state, float* rates) {
+    if (!m || !state || !rates) return MBG_ERR_ARG;
+    for (int i = 0; i < m->cm.N; i++) rates[i] = mbg::node_rate(m->cm, i, state);
+    return MBG_OK;
+}
+
 mbg_status mbg_model_generate_source(const mbg_model* m, int trace, char* buf, size_t cap, size_t* len) {
     if (!m) return MBG_ERR_ARG;
     const mbg::CompiledModel& cm = m->cm;
