@@ -137,6 +137,36 @@ def surrogate_bench():
     }
 
 
+def ring_bench(ctx, sizes=(100, 500), cpu=True):
+    """The ring models of the reference's own generator (data/generate-synth.py defaults: 5
+    readouts, 1,000,000 trajectories, time_tick 5, max_time 100, signal length int(sqrt(N)),
+    so every node reads 2 int(sqrt(N)) + 1 others), at its default size and near its default
+    512-node build limit; the CPU oracle runs a bounded sample of the same model."""
+    from oracle.oracle import OracleModel
+    from upmabossg_amd import Model, synth
+
+    out = []
+    for n in sizes:
+        b, c = synth.ring_network(n, sample_count=1000000)
+        m = Model(b, c)
+        ctx.run(m, count=65536, n_trajectories=65536, fetch=False)  # compile, buffers
+        t0 = time.perf_counter()
+        ctx.run(m, fetch=False)
+        wall = time.perf_counter() - t0
+        i = ctx.info()
+        e = {"nodes": n, "inputs_per_node": 2 * int(n ** 0.5) + 1, "trajectories": int(i.simulated),
+             "transitions": int(i.transitions), "kernel_ms": i.kernel_ms, "wall_ms": wall * 1e3,
+             "transitions_per_s": i.transitions / wall}
+        if cpu:
+            cores = min(16, os.cpu_count() or 1)
+            cnt = 16384 if n <= 100 else 2048  # a few seconds of CPU work
+            r = OracleModel(b, c).run(count=cnt, n_trajectories=1000000, threads=cores, record_fixed=False)
+            e["cpu_transitions_per_s"] = r.flips / r.seconds
+            e["cpu_sample"] = f"trajectory ids 0-{cnt - 1}, {cores} threads, {r.seconds:.1f} s"
+        out.append(e)
+    return out
+
+
 def cpu_baseline(net, count=4096):
     from oracle.oracle import OracleModel
 
@@ -157,6 +187,7 @@ def main():
     ap.add_argument("--nodes", type=int, default=1000)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-ring", action="store_true", help="skip the reference generator's ring models")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the untuned launch shape (and the MBG_TUNE_* environment knobs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -208,6 +239,13 @@ def main():
             except Exception as e:  # reported, never fatal to the headline line
                 sweep = {"error": f"{type(e).__name__}: {e}"}
             print(f"sweep: {sweep}", file=sys.stderr, flush=True)
+        ring = None
+        if not args.no_ring:
+            try:
+                ring = ring_bench(ctx, cpu=not args.no_cpu)
+            except Exception as e:  # reported, never fatal to the headline line
+                ring = {"error": f"{type(e).__name__}: {e}"}
+            print(f"ring: {ring}", file=sys.stderr, flush=True)
 
     kern_s = sum(kernel_ms) / len(kernel_ms) / 1e3
     traffic = traffic_from_profile()
@@ -260,6 +298,8 @@ def main():
     }
     if sweep is not None:
         out["sweep"] = sweep
+    if ring is not None:
+        out["reference_generator_rings"] = ring
     if not args.no_surrogate:
         try:
             out["surrogate"] = surrogate_bench()
