@@ -274,6 +274,7 @@ public:
 private:
     std::vector<Tk> toks_;
     size_t p_ = 0;
+    int nest_ = 0;  // expression() calls in progress
     ParsedModel& m_;
 
     const Tk& at() const { return toks_[p_]; }
@@ -330,6 +331,7 @@ private:
                 AstPtr a = leaf(Ast::Group);
                 a->kids.push_back(expression(0));
                 take(T::RParen, "')'");
+                bounded(a);
                 return a;
             }
             case T::Minus:
@@ -338,6 +340,7 @@ private:
                 AstPtr a = leaf(Ast::Unary);
                 a->oper = t.type == T::Minus ? Ast::Sub : t.type == T::Plus ? Ast::Pos : Ast::Not;
                 a->kids.push_back(expression(kUnaryPower));
+                bounded(a);
                 return a;
             }
             default:
@@ -346,8 +349,23 @@ private:
         }
     }
 
+    // Expression trees are walked recursively (evaluation, code generation, destruction), so
+    // their height is bounded.
+    static constexpr int kMaxHeight = 10000;  // Bison's default parser stack (YYMAXDEPTH)
+    AstPtr& bounded(AstPtr& a) {
+        for (const auto& k : a->kids) a->height = std::max(a->height, k->height + 1);
+        if (a->height > kMaxHeight) fail("expression nested too deeply (more than " + std::to_string(kMaxHeight) + " levels)");
+        return a;
+    }
+
     // Pratt loop: binary operators are left-associative, "?:" right-associative.
     AstPtr expression(int min_power) {
+        if (++nest_ > kMaxHeight) fail("expression nested too deeply (more than " + std::to_string(kMaxHeight) + " levels)");
+        AstPtr e = expression_body(min_power);
+        nest_--;
+        return e;
+    }
+    AstPtr expression_body(int min_power) {
         AstPtr lhs = prefix();
         for (;;) {
             const T op = at().type;
@@ -360,6 +378,7 @@ private:
                 a->kids.push_back(expression(0));
                 take(T::Colon, "':'");
                 a->kids.push_back(expression(0));
+                bounded(a);
                 lhs = std::move(a);
                 continue;
             }
@@ -367,6 +386,7 @@ private:
             a->oper = infix_oper(op);
             a->kids.push_back(std::move(lhs));
             a->kids.push_back(expression(pw));
+            bounded(a);
             lhs = std::move(a);
         }
     }

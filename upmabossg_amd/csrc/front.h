@@ -25,6 +25,7 @@ struct Ast {
     float number = 0.f;
     std::string text;  // node / $variable / @alias name
     std::vector<std::unique_ptr<Ast>> kids;
+    int height = 1;  // 1 + the tallest kid (bounded by the parser: later passes recurse)
 };
 using AstPtr = std::unique_ptr<Ast>;
 
