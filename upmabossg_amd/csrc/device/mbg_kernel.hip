@@ -804,7 +804,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 
         // ---- flip selection: group sums, then the active bits of the group's words ----
         const u64 target = mbg_flip_target(dr2, total);
-        u64 acc = 0;
+        u64 acc = 0, sgs = 1;
         int gsel = NSUM - 1;
         bool found = false;
 #pragma unroll
@@ -813,20 +813,33 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             if (!found) {
                 if (acc + sg > target) {
                     gsel = g;
+                    sgs = sg;
                     found = true;
                 } else {
                     acc += sg;
                 }
             }
         }
-        u64 rem = target - acc;
+        // In the group, the selected node is the first (in node order) whose inclusive prefix
+        // of rates exceeds rem = target - acc.  Equivalently it is the first from the top whose
+        // inclusive suffix reaches sgs - rem (the group sum minus rem): each lane searches from
+        // the nearer end, which halves the rounds the slowest lane of the wave needs.  Both
+        // directions share one loop: `lim` is rem from the bottom (hit when rate > lim) and
+        // sgs - rem - 1 from the top (hit when rate >= sgs - rem), and from the top the active
+        // bits are read bit-reversed so that the lowest set bit is always the next candidate.
+        const u64 rem0 = target - acc;
+        const bool down = found && rem0 >= (sgs >> 1);
+        u64 lim = down ? sgs - rem0 - 1ull : rem0;
         // guard for an inconsistent group (never taken when the sums are exact)
         u32 j = (u32)(gsel * MBG_SUMW * 64);
         u64 r_sel = 0, xw = LANE_X(j >> 6);
         bool hit = false;
-        for (int q = 0; q < MBG_SUMW && !hit; q++) {
+        for (int q0 = 0; q0 < MBG_SUMW && !hit; q0++) {
+            const int q = down ? MBG_SUMW - 1 - q0 : q0;
             const u32 wi = (u32)(gsel * MBG_SUMW + q);
-            u64 m = LANE_A(wi);
+            const u64 am = LANE_A(wi);
+            u64 m = down ? (((u64)__builtin_bitreverse32((u32)am) << 32) | (u64)__builtin_bitreverse32((u32)(am >> 32)))
+                         : am;
             const u64 xq = LANE_X(wi);
 #if MBG_GENERIC
             const u64 gm = TGEN(wi);
@@ -838,7 +851,8 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 u64 rr[MBG_SCAN], mm = m;
 #pragma unroll
                 for (int c = 0; c < MBG_SCAN; c++) {
-                    bb[c] = mm ? (u32)__builtin_ctzll(mm) : 64u;
+                    const u32 t = mm ? (u32)__builtin_ctzll(mm) : 64u;
+                    bb[c] = t < 64u ? (down ? 63u - t : t) : 64u;
                     mm = mm ? (mm & (mm - 1ull)) : 0ull;
                     const u32 b = bb[c] < 64u ? bb[c] : 0u;
                     rr[c] = bb[c] < 64u ? TRATE(2 * (wbase + b) + ((xq >> b) & 1ull)) : 0ull;
@@ -852,13 +866,13 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #pragma unroll
                 for (int c = 0; c < MBG_SCAN; c++) {
                     if (!hit && bb[c] < 64u) {
-                        if (rem < rr[c]) {
+                        if (lim < rr[c]) {
                             j = wbase + bb[c];
                             r_sel = rr[c];
                             xw = xq;
                             hit = true;
                         } else {
-                            rem -= rr[c];
+                            lim -= rr[c];
                         }
                     }
                 }
