@@ -32,6 +32,13 @@ def _variants():
     out.append((b, c, 2000, None))
     b, c = model_text("metastasis")
     out.append((b, c, 500, None))
+    # nodes reading more than 20 nodes (logic programs) in classes shared with table nodes
+    b, c = synth.ring_network(100, steps_to_fixed_point=50, sample_count=1000, time_tick=5.0, max_time=100.0)
+    out.append((b, c, 1000, None))
+    from tests.wide_models import wide_model
+
+    b, c = wide_model(seed=5, sample_count=800)
+    out.append((b, c, 800, None))
     return out
 
 
@@ -68,3 +75,12 @@ def test_sweep_matches_oracle_per_variant(gpu_ctx):
             assert np.allclose(gpu_ctx.variant_marginals(v), want, rtol=1e-12, atol=0)
         total += r.flips
     assert info.transitions == total
+
+
+def test_sweep_refuses_more_than_20_readouts(gpu_ctx):
+    from upmabossg_amd import MbgError
+
+    net = synth.random_network(40, 2, 3, seed=3, readouts=24, sample_count=100)
+    with pytest.raises(MbgError) as e:
+        gpu_ctx.run_variants([(Model(net.bnd(), net.cfg()), 100, 1, 0)])
+    assert e.value.status == 2 and "at most 20" in str(e.value)
