@@ -204,3 +204,24 @@ def test_cli_variants_file_bytes(tmp_path):
     assert text.stdout == "".join(blocks)
     assert (tmp_path / "out_wt_probtraj.csv").read_text() == (tmp_path / "plain_probtraj.csv").read_text()
     assert (tmp_path / "out_wt_fp.csv").read_text() == (tmp_path / "plain_fp.csv").read_text()
+
+
+def test_cli_variants_more_than_20_readouts(tmp_path):
+    """-v on a model with 24 readouts (sparse window cells): the variants run one by one (sweeps
+    keep dense tables), with the same counts, seeds and draws, and give the oracle's bytes."""
+    from upmabossg_amd import Model
+
+    bnd, cfg = _readouts24()
+    text = "variant wt { }\nvariant ko { n3.knockout; n30.rate_up *= 2; sample_count = 1000; }\n"
+    bp, cp, vp = tmp_path / "m.bnd", tmp_path / "m.cfg", tmp_path / "v.txt"
+    bp.write_text(bnd)
+    cp.write_text(cfg)
+    vp.write_text(text)
+    print("case: CLI variants readouts24", flush=True)
+    run = subprocess.run([CLI_PATH, "-o", str(tmp_path / "out"), "-v", str(vp), str(bp), str(cp)],
+                         capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    for name, model, n, seed in Model(bnd, cfg).variants(text):
+        want = Expected(bnd, cfg, variant=model.get_variant(), n=n, seed=seed)
+        assert (tmp_path / f"out_{name}_probtraj.csv").read_text() == want.probtraj_csv(), name
+        assert (tmp_path / f"out_{name}_fp.csv").read_text() == want.fp_csv(), name

@@ -60,8 +60,39 @@ bool load_variants(const mbg_model* base, const std::string& path, std::vector<m
     return true;
 }
 
+// More than 20 readouts: sweeps keep dense tables only, so each variant is a plain run (same
+// trajectory count, seed and draws as in a sweep).
+int run_variants_one_by_one(mbg_ctx* ctx, const std::vector<mbg_variant>& vs, const std::vector<std::string>& names,
+                            const std::string& prefix) {
+    for (size_t i = 0; i < vs.size(); i++) {
+        mbg_run_params p = {};
+        p.n_trajectories = vs[i].n_trajectories;
+        p.seed = vs[i].seed;
+        p.use_model_seed = vs[i].use_model_seed;
+        p.variant_id = vs[i].variant_id;
+        if (mbg_run(ctx, vs[i].model, &p) != MBG_OK) {
+            std::cerr << mbg_last_error(ctx) << std::endl;
+            return 1;
+        }
+        const mbg_model* m = vs[i].model;
+        mbg_status st;
+        if (!prefix.empty()) {
+            st = mbg_write_csv(ctx, m, (prefix + "_" + names[i]).c_str());
+        } else {
+            std::cout << "variant " << names[i] << std::endl;
+            st = mbg_write_text(ctx, m, "-");
+        }
+        if (st != MBG_OK) {
+            std::cerr << mbg_last_error(ctx) << std::endl;
+            return 1;
+        }
+    }
+    return 0;
+}
+
 int run_variants(mbg_ctx* ctx, const mbg_model* base, const std::vector<mbg_variant>& vs,
                  const std::vector<std::string>& names, const std::string& prefix) {
+    if (mbg_model_num_readouts(base) > 20) return run_variants_one_by_one(ctx, vs, names, prefix);
     const int count = (int)vs.size();
     const double budget = 32.0 * (1ull << 30);
     const int group = std::max(1, std::min(256, (int)(budget / table_bytes(base))));
