@@ -18,6 +18,12 @@
  * with the documented deviations of this project (FP64 clock, Philox4x32-10
  * keyed by trajectory id, exact fixed-point rate sums, fixed-point window
  * accumulators) so that the GPU kernel can be replayed bit-exactly.
+ *
+ * Two extensions beyond the reference's limits follow the same semantics: any
+ * number of inputs per fast node (expressions are interpreted, not tabulated),
+ * and more than 20 readouts, where the window cells and final states are kept
+ * in per-thread maps instead of dense tables (pinned against the dense path by
+ * marginalising a 24-readout run onto 20 readouts, tests/test_sparse_oracle.py).
  */
 #pragma once
 #include <stddef.h>
