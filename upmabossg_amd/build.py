@@ -5,7 +5,7 @@
 * Embeds the device sources (csrc/device/*) as C strings for hipRTC.
 * Compiles the host library with hipcc (host code only; the trajectory kernel is
   compiled at run time by hipRTC for gfx950) and links the CLI against it.
-* Compile-checks the generated kernel for gfx950 with hipcc for six shapes, so a
+* Compile-checks the generated kernel for gfx950 with hipcc for eight shapes, so a
   device-code error shows up at build time rather than at the first run.
 * Compiles the surrogate MLP library (libmbg_mlp.so, BF16 MFMA kernels) for gfx950.
 * Builds the CPU oracle (oracle/liboracle.so, test infrastructure).
@@ -91,7 +91,9 @@ def check_kernel(verbose=False):
                   (32, 256, 0, 0, 0, 0, 4, 4, 8, 0, 1, 1, 1, 1, 0, 0, 1),
                   (16, 448, 0, 1, 0, 1, 4, 8, 4, 1, 1, 0, 2, 0, 0, 0, 0),
                   (2, 256, 0, 1, 1, 1, 6, 2, 4, 1, 0, 0, 1, 0, 0, 1, 1),
-                  (200, 64, 1, 0, 1, 0, 3, 2, 4, 0, 0, 0, 1, 0, 0, 0, 1)]
+                  (200, 64, 1, 0, 1, 0, 3, 2, 4, 0, 0, 0, 1, 0, 0, 0, 1),
+                  (79, 64, 0, 0, 0, 0, 3, 1, 4, 2, 0, 0, 5, 0, 1, 0, 0),
+                  (64, 128, 0, 0, 0, 0, 4, 2, 4, 2, 1, 1, 4, 0, 0, 0, 0)]
         for nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp, pg in shapes:
             src = os.path.join(td, f"k{nw}_{disc}_{trace}_{multi}_{cp}_{sp}_{pg}.hip")
             with open(src, "w") as f:

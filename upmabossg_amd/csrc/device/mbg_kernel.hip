@@ -10,10 +10,12 @@
 //   MBG_MAX_IN      largest number of inputs of a node (gathers are unrolled up to 6)
 //   MBG_DEP_BATCH   dependents evaluated per round of independent table reads
 //   MBG_SCAN        candidate nodes per round of the in-word flip search
-//   MBG_WSUM_REG    1: the word rate sums live in VGPRs (more lanes fit in LDS), 0: in LDS
+//   MBG_WSUM_REG    1: the word rate sums live in VGPRs (more lanes fit in LDS), 0: in LDS,
+//                   2: in LDS, with sums of groups of MBG_SUMW words in VGPRs (wide models)
 //   MBG_RATE32      1: every fixed-point rate fits 32 bits, the rate table is u32
 //   MBG_MULTI       1: emit the variant-sweep kernel instead of the single-model one
-//   MBG_SUMW        64-bit words per rate-sum group (register sums only; divides MBG_NW)
+//   MBG_SUMW        64-bit words per register rate-sum group (divides MBG_NW when
+//                   MBG_WSUM_REG is 1; the last group may be short when it is 2)
 //   MBG_COMPACT     1: 8-byte node records and 16-bit dependents (small models, no generic rates)
 //   MBG_WSMAD       1: every rate is below 2^31 (register sum updates by multiply-add)
 //   MBG_SPARSE      1: more than 20 readouts, window cells and final states in a hash table
@@ -231,14 +233,18 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
 #define LANE_A(w) lane[(1 * MBG_NW + (w)) * MBG_BLOCK + tid]
 #if MBG_WSUM_REG
 // rate sums of groups of MBG_SUMW words in registers: every access is an unrolled select
-// over the MBG_NW / MBG_SUMW groups
+// over the groups (at most 16)
+#if MBG_WSUM_REG == 2
+#define NSUM ((MBG_NW + MBG_SUMW - 1) / MBG_SUMW)
+#else
 #define NSUM (MBG_NW / MBG_SUMW)
+#endif
 #define LANE_S(g) ws[g]
 #if MBG_WSMAD
 // every rate change fits an i32: ws[i] += [i == g] * d as one 32 x 32 + 64 multiply-add
-#define WS_ADD(w, d)                                                       \
+#define WS_GADD(g, d)                                                      \
     do {                                                                   \
-        const u32 oh_ = 1u << ((w) / MBG_SUMW);                            \
+        const u32 oh_ = 1u << (g);                                         \
         const long long d_ = (long long)(int)(u32)(d);                     \
         _Pragma("unroll") for (int i_ = 0; i_ < NSUM; i_++) {              \
             u32 f_ = (oh_ >> i_) & 1u;                                     \
@@ -247,12 +253,25 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
         }                                                                  \
     } while (0)
 #else
-#define WS_ADD(w, d)                                                       \
+#define WS_GADD(g, d)                                                      \
     do {                                                                   \
-        const u32 g_ = (w) / MBG_SUMW;                                     \
+        const u32 g_ = (g);                                                \
         const u64 d_ = (d);                                                \
         _Pragma("unroll") for (int i_ = 0; i_ < NSUM; i_++) ws[i_] += g_ == (u32)i_ ? d_ : 0ull; \
     } while (0)
+#endif
+#if MBG_WSUM_REG == 2
+// the word sums themselves in LDS (lane-interleaved like x and act)
+#define LANE_W(w) lane[(2 * MBG_NW + (w)) * MBG_BLOCK + tid]
+#define WS_ADD(w, d)                                                       \
+    do {                                                                   \
+        const u32 w2_ = (w);                                               \
+        const u64 dd_ = (d);                                               \
+        LANE_W(w2_) = LANE_W(w2_) + dd_;                                   \
+        WS_GADD(w2_ / MBG_SUMW, dd_);                                      \
+    } while (0)
+#else
+#define WS_ADD(w, d) WS_GADD((w) / MBG_SUMW, d)
 #endif
 #else
 #undef MBG_SUMW
@@ -631,9 +650,13 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #pragma unroll
                         for (int q = 0; q < MBG_SUMW; q++) {
                             const int w = g * MBG_SUMW + q;
+                            if (MBG_WSUM_REG == 2 && w >= MBG_NW) continue;  // short last group
                             LANE_X(w) = st[w];
                             LANE_A(w) = st[MBG_NW + w];
                             gs += st[2 * MBG_NW + w];
+#if MBG_WSUM_REG == 2
+                            LANE_W(w) = st[2 * MBG_NW + w];
+#endif
                         }
                         LANE_S(g) = gs;
                         total += gs;
@@ -827,16 +850,46 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         // directions share one loop: `lim` is rem from the bottom (hit when rate > lim) and
         // sgs - rem - 1 from the top (hit when rate >= sgs - rem), and from the top the active
         // bits are read bit-reversed so that the lowest set bit is always the next candidate.
+#if MBG_WSUM_REG == 2
+        // second level: the word sums of the selected group, in LDS
+        u32 w0 = (u32)(gsel * MBG_SUMW);
+        {
+            bool f2 = false;
+            u64 sw_sel = 1;
+            u32 wsel = w0;
+#pragma unroll
+            for (int q = 0; q < MBG_SUMW; q++) {
+                const u32 w = (u32)(gsel * MBG_SUMW + q);
+                if (w < (u32)MBG_NW && !f2) {
+                    const u64 sw = LANE_W(w);
+                    if (acc + sw > target) {
+                        wsel = w;
+                        sw_sel = sw;
+                        f2 = true;
+                    } else {
+                        acc += sw;
+                    }
+                }
+            }
+            found = found && f2;
+            sgs = sw_sel;
+            w0 = wsel;
+        }
+#define SEARCH_W 1
+#else
+        const u32 w0 = (u32)(gsel * MBG_SUMW);
+#define SEARCH_W MBG_SUMW
+#endif
         const u64 rem0 = target - acc;
         const bool down = found && rem0 >= (sgs >> 1);
         u64 lim = down ? sgs - rem0 - 1ull : rem0;
         // guard for an inconsistent group (never taken when the sums are exact)
-        u32 j = (u32)(gsel * MBG_SUMW * 64);
+        u32 j = w0 * 64u;
         u64 r_sel = 0, xw = LANE_X(j >> 6);
         bool hit = false;
-        for (int q0 = 0; q0 < MBG_SUMW && !hit; q0++) {
-            const int q = down ? MBG_SUMW - 1 - q0 : q0;
-            const u32 wi = (u32)(gsel * MBG_SUMW + q);
+        for (int q0 = 0; q0 < SEARCH_W && !hit; q0++) {
+            const int q = down ? SEARCH_W - 1 - q0 : q0;
+            const u32 wi = w0 + (u32)q;
             const u64 am = LANE_A(wi);
             u64 m = down ? (((u64)__builtin_bitreverse32((u32)am) << 32) | (u64)__builtin_bitreverse32((u32)(am >> 32)))
                          : am;

@@ -389,7 +389,8 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
     return c->modules.emplace(sh, m).first->second;
 }
 
-size_t lane_lds_bytes(int nw, int block, int wsum_reg) { return (size_t)(wsum_reg ? 2 : 3) * nw * block * 8; }
+// LDS of the lanes' state: x and act, plus the word rate sums unless they are in VGPRs only
+size_t lane_lds_bytes(int nw, int block, int wsum_reg) { return (size_t)(wsum_reg == 1 ? 2 : 3) * nw * block * 8; }
 
 int env_int(const char* name, int dflt) {
     const char* e = std::getenv(name);
@@ -398,13 +399,22 @@ int env_int(const char* name, int dflt) {
 
 // Word rate sums live in VGPRs (an unrolled select over at most 16 sums per update) when
 // the model has at most 16 words, or up to 32 words in groups of two (an even word count).
-// Otherwise they take a third LDS array beside x and act.
-int default_wsum_reg(int nw) { return env_int("MBG_TUNE_WSUM_REG", (nw <= 16 || (nw <= 32 && nw % 2 == 0)) ? 1 : 0) ? 1 : 0; }
+// Otherwise they take a third LDS array beside x and act, summed again in at most 16 group
+// sums in VGPRs (mode 2) so that a selection does not scan every word.
+bool reg_sums_fit(int nw) { return nw <= 16 || (nw <= 32 && nw % 2 == 0); }
+// 1: word sums in VGPRs; 2 (wider models): word sums in LDS under at most 16 group sums in
+// VGPRs, so a selection scans the groups and then one group's words; 0: LDS only.
+int default_wsum_reg(int nw) {
+    int v = env_int("MBG_TUNE_WSUM_REG", reg_sums_fit(nw) ? 1 : 2);
+    if (v == 1 && !reg_sums_fit(nw)) v = 2;
+    return v < 0 || v > 2 ? 0 : v;
+}
 
 // Words per register rate-sum group: fewer, wider groups make each sum update cheaper (an
 // unrolled select over the groups) at the cost of scanning more act words per selection.
 int default_sumw(int nw, int wsum_reg) {
     if (!wsum_reg) return 1;
+    if (wsum_reg == 2) return (nw + 15) / 16;  // at most 16 groups, the last one possibly short
     int sw = env_int("MBG_TUNE_SUMW", nw > 16 ? 2 : 1);
     if (sw < 1) sw = 1;
     while (sw > 1 && nw % sw) sw /= 2;
@@ -964,7 +974,7 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
     {
         std::vector<Shape> cands;
         auto add = [&](int compact, int wreg, int tables, int below) {
-            if (wreg && !(cm.NW <= 16 || (cm.NW <= 32 && cm.NW % 2 == 0))) return;
+            if (wreg == 1 && !reg_sums_fit(cm.NW)) return;
             if (compact && !cfit) return;
             const uint32_t blob = pack_blob(cm, rt, r32, compact != 0).lay.bytes;
             int waves = 0;
@@ -973,7 +983,7 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
             Shape sh = base;
             sh.compact = compact;
             sh.wsum_reg = wreg;
-            sh.sumw = wreg ? default_sumw(cm.NW, 1) : 1;
+            sh.sumw = default_sumw(cm.NW, wreg);
             sh.wsmad = wreg && r31 ? 1 : 0;
             sh.tables_lds = tables;
             sh.block = block;
@@ -981,7 +991,7 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
         };
         add(base.compact, base.wsum_reg, !base.tables_lds, 1 << 30);
         add(!base.compact, base.wsum_reg, base.tables_lds, 1 << 30);
-        add(base.compact, !base.wsum_reg, base.tables_lds, 1 << 30);
+        add(base.compact, base.wsum_reg ? 0 : (reg_sums_fit(cm.NW) ? 1 : 2), base.tables_lds, 1 << 30);
         {
             const uint32_t blob = pack_blob(cm, rt, r32, base.compact != 0).lay.bytes;
             const size_t lds = lane_lds_bytes(cm.NW, base.block, base.wsum_reg) + (base.tables_lds ? blob : 0);
