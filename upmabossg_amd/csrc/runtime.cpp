@@ -992,6 +992,8 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
         add(base.compact, base.wsum_reg, !base.tables_lds, 1 << 30);
         add(!base.compact, base.wsum_reg, base.tables_lds, 1 << 30);
         add(base.compact, base.wsum_reg ? 0 : (reg_sums_fit(cm.NW) ? 1 : 2), base.tables_lds, 1 << 30);
+        // 17-32 words: the two-level sums search one word instead of a two-word group
+        if (base.wsum_reg == 1 && cm.NW > 16) add(base.compact, 2, base.tables_lds, 1 << 30);
         {
             const uint32_t blob = pack_blob(cm, rt, r32, base.compact != 0).lay.bytes;
             const size_t lds = lane_lds_bytes(cm.NW, base.block, base.wsum_reg) + (base.tables_lds ? blob : 0);
