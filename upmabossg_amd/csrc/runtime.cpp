@@ -61,6 +61,45 @@ void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw Error(MBG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Timing events, destroyed on every exit path.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        hip_check(hipEventCreate(&a), "hipEventCreate");
+        if (hipEventCreate(&b) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            throw Error(MBG_ERR_HIP, "hipEventCreate failed");
+        }
+    }
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+};
+
+// A stream capture and what it produces: an error thrown while capturing ends the capture
+// (so the stream is usable again) and every event, graph and executable graph is released.
+struct CaptureScope {
+    hipStream_t stream = nullptr;
+    bool capturing = false;
+    std::vector<hipEvent_t> events;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~CaptureScope() {
+        if (capturing) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+        for (hipEvent_t e : events)
+            if (e) (void)hipEventDestroy(e);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
 void copy_err(char* err, size_t n, const std::string& s) {
     if (err && n) {
         std::strncpy(err, s.c_str(), n - 1);
@@ -826,9 +865,8 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     D.sigma = rt.sigma;
     D.compat = p->compat_batch_resume ? 1 : 0;
 
-    hipEvent_t e0, e1;
-    hip_check(hipEventCreate(&e0), "hipEventCreate");
-    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    EventPair ev;
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     float sim_ms = 0.f;
     for (;;) {
         d_prob.zero((size_t)W * S, s);
@@ -877,16 +915,11 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
         uint64_t miss = 0;
         hip_check(hipMemcpy(&miss, B.cell_miss.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
         if (miss == 0) break;
-        if (cell_cap >= kMaxCells) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+        if (cell_cap >= kMaxCells)
             throw Error(MBG_ERR_LIMIT, "sparse window table full: more distinct (window, state) cells than " +
                                            std::to_string(kMaxCells / 2));
-        }
         cell_cap = std::min<uint64_t>(cell_cap * 4, kMaxCells);  // repeat the run with a larger table
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 
     c->prob.resize((size_t)W * S);  // filled from the device by the first getter that needs it
     c->prob_on_host = false;
@@ -1245,11 +1278,13 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         hip_check(hipStreamCreate(&st), "hipStreamCreate");
         c->class_streams.push_back(st);
     }
-    std::vector<hipEvent_t> evs(sw->classes.size() + 1);
+    CaptureScope cap;
+    cap.stream = main;
+    cap.events.assign(sw->classes.size() + 1, nullptr);
+    std::vector<hipEvent_t>& evs = cap.events;
     for (auto& e : evs) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
     hip_check(hipStreamBeginCapture(main, hipStreamCaptureModeGlobal), "hipStreamBeginCapture");
+    cap.capturing = true;
     hip_check(hipEventRecord(evs[0], main), "hipEventRecord");
     for (size_t ci = 0; ci < sw->classes.size(); ci++) {
         SweepClass& C = *sw->classes[ci];
@@ -1295,23 +1330,17 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
                                         1, 0, main, margs, nullptr),
                   "launch mbg_marginals_batch");
     }
-    hip_check(hipStreamEndCapture(main, &graph), "hipStreamEndCapture");
-    hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    cap.capturing = false;
+    hip_check(hipStreamEndCapture(main, &cap.graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     const auto t1 = std::chrono::steady_clock::now();
-    hipEvent_t e0, e1;
-    hip_check(hipEventCreate(&e0), "hipEventCreate");
-    hip_check(hipEventCreate(&e1), "hipEventCreate");
-    hip_check(hipEventRecord(e0, main), "hipEventRecord");
-    hip_check(hipGraphLaunch(exec, main), "hipGraphLaunch");
-    hip_check(hipEventRecord(e1, main), "hipEventRecord");
+    EventPair ev;
+    hip_check(hipEventRecord(ev.a, main), "hipEventRecord");
+    hip_check(hipGraphLaunch(cap.exec, main), "hipGraphLaunch");
+    hip_check(hipEventRecord(ev.b, main), "hipEventRecord");
     hip_check(hipStreamSynchronize(main), "variant sweep");
     float ms = 0.f;
-    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    for (auto& e : evs) (void)hipEventDestroy(e);
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    hip_check(hipEventElapsedTime(&ms, ev.a, ev.b), "hipEventElapsedTime");
 
     mbg_sweep_info& SI = sw->info;
     SI = mbg_sweep_info{};
