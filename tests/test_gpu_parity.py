@@ -297,3 +297,37 @@ def test_wide_logic_programs_bit_exact(gpu_ctx, seed, discrete):
 
     b, c = wide_model(seed=seed, discrete=discrete)
     _compare(gpu_ctx, b, c, count=2000)
+
+
+@pytest.mark.parametrize("discrete,compat", [(False, False), (True, False), (False, True)])
+def test_all_extensions_together_bit_exact(gpu_ctx, discrete, compat):
+    """One model with every extension at once: 24 readouts (sparse cells), wide nodes (logic
+    programs), a generic rate expression, XOR, variant multipliers and an istate override,
+    in continuous and discrete time and with the batch-resume option."""
+    from tests.wide_models import wide_model
+
+    b, c = wide_model(n=48, wide=8, seed=9, discrete=discrete, sample_count=1500, max_time=12.0, time_tick=0.5)
+    b += ("node gx { logic = w0 ^ w1; rate_up = (w2 & !w3) ? 1.5 : 0.25; rate_down = w4 ? 0.5 : 2; }\n")
+    for i in range(6, 24):  # nodes 0-23 and gx are readouts
+        c = c.replace(f"w{i}.is_internal = 1;", "")
+    m, o = Model(b, c), OracleModel(b, c)
+    assert m.num_readouts > 20
+    N = m.num_nodes
+    rng = np.random.default_rng(4)
+    mu = np.exp(rng.normal(0, 0.4, N)).astype(np.float32)
+    md = np.exp(rng.normal(0, 0.4, N)).astype(np.float32)
+    ist = rng.uniform(0, 1, N).astype(np.float32)
+    m.set_variant(mu, md, ist)
+    o.set_variant(mu, md, ist)
+    _flush(f"case: all extensions discrete={discrete} compat={compat}")
+    g = gpu_ctx.run(m, n_trajectories=1500, count=1500, trace=True, compat_batch_resume=compat)
+    r = o.run(count=1500, n_trajectories=1500, threads=16, record_summaries=True, compat_batch_resume=compat)
+    assert np.array_equal(g.summaries[:, 0], r.summaries["flips"])
+    assert np.array_equal(g.summaries[:, 1], r.summaries["flip_hash"])
+    assert g.transitions == r.flips
+    for a, bb in zip(g.cells, r.cells):
+        assert np.array_equal(a, bb)
+    for a, bb in zip(g.final_cells, r.final_cells):
+        assert np.array_equal(a, bb)
+    assert np.array_equal(g.th_raw, r.th)
+    assert np.array_equal(g.fixed_states, r.fixed_states) and np.array_equal(g.fixed_counts, r.fixed_counts)
