@@ -20,6 +20,7 @@
 //   MBG_WSMAD       1: every rate is below 2^31 (register sum updates by multiply-add)
 //   MBG_SPARSE      1: more than 20 readouts, window cells and final states in a hash table
 //   MBG_PROG        1: some fast node reads more than 20 nodes (logic program, no truth table)
+//   MBG_TH_LDS      1: per-block partial sums of the window entropies in LDS (single runs)
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -76,6 +77,13 @@ struct mbg_node {
 #endif
 #ifndef MBG_PROG
 #define MBG_PROG 0
+#endif
+#ifndef MBG_TH_LDS
+#define MBG_TH_LDS 0
+#endif
+#if MBG_MULTI  // sweeps: waves of one block work on different variants
+#undef MBG_TH_LDS
+#define MBG_TH_LDS 0
 #endif
 #define PROG_IN 255u  // in_cnt of a node whose logic is a program
 // Projected (readout) state: up to 20 bits in a dense table, up to 63 with MBG_SPARSE.
@@ -595,7 +603,20 @@ __device__ __forceinline__ void flush_cell(const mbg_desc& D, int w, sidx s, u64
 
 // Simulates trajectories of one model (descriptor D) on this wave until its work queue is
 // empty and every lane of the wave is idle.
-__device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* tb, u64* lane) {
+// Window entropy flush: every trajectory adds to th[w] for every window it visits, so the W
+// cells are the hottest addresses of the run; with MBG_TH_LDS the block sums them in LDS
+// (thl, W entries) and flushes once at its end (exact: integer sums in any order).
+__device__ __forceinline__ void flush_th(const mbg_desc& D, u64* thl, int w, u64 v) {
+#if MBG_TH_LDS
+    (void)D;
+    atomicAdd(thl + w, v);
+#else
+    (void)thl;
+    atomicAdd(D.th + w, v);
+#endif
+}
+
+__device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* tb, u64* lane, u64* thl) {
     const u32 tid = threadIdx.x;
     const u32 lid = tid & 63u;
     Lane L{D, tb, lane, tid};
@@ -742,7 +763,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             if (w != cur_w) {
                 if (cur_w >= 0) {
                     flush_cell(D, cur_w, acc_s, acc_p);
-                    if (acc_th) atomicAdd(D.th + cur_w, acc_th);
+                    if (acc_th) flush_th(D, thl, cur_w, acc_th);
                 }
                 cur_w = w;
                 acc_p = 0;
@@ -768,7 +789,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 if (ww != cur_w) {
                     if (cur_w >= 0) {
                         flush_cell(D, cur_w, acc_s, acc_p);
-                        if (acc_th) atomicAdd(D.th + cur_w, acc_th);
+                        if (acc_th) flush_th(D, thl, cur_w, acc_th);
                     }
                     cur_w = ww;
                     acc_p = 0;
@@ -792,7 +813,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             // ---- trajectory end: flush, final state, fixed point, summary ----
             if (cur_w >= 0) {
                 flush_cell(D, cur_w, acc_s, acc_p);
-                if (acc_th) atomicAdd(D.th + cur_w, acc_th);
+                if (acc_th) flush_th(D, thl, cur_w, acc_th);
             }
 #if MBG_SPARSE
             sparse_add(D, ((u64)D.W << D.K) | s, 1ull);
@@ -1020,9 +1041,23 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
         for (u32 i = threadIdx.x; i < words; i += MBG_BLOCK) smem[i] = src[i];
         __syncthreads();
     }
-    simulate_variant(D, (const char*)smem, smem + D.lay.bytes / 8);
+    const char* tb = (const char*)smem;
+    u64* lane = smem + D.lay.bytes / 8;
 #else
-    simulate_variant(D, D.blob, smem);
+    const char* tb = D.blob;
+    u64* lane = smem;
+#endif
+#if MBG_TH_LDS
+    // the block's window-entropy partial sums follow the lanes' state
+    u64* thl = lane + (MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
+    for (u32 i = threadIdx.x; i < (u32)D.W; i += MBG_BLOCK) thl[i] = 0;
+    __syncthreads();
+    simulate_variant(D, tb, lane, thl);
+    __syncthreads();  // every wave of the block has left its loop
+    for (u32 i = threadIdx.x; i < (u32)D.W; i += MBG_BLOCK)
+        if (thl[i]) atomicAdd(D.th + i, thl[i]);
+#else
+    simulate_variant(D, tb, lane, nullptr);
 #endif
 }
 #else
@@ -1040,7 +1075,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
         if (v >= nv) v -= nv;
         v = __builtin_amdgcn_readfirstlane(v);
         const mbg_desc& D = descs[v];
-        simulate_variant(D, D.blob, smem);
+        simulate_variant(D, D.blob, smem, nullptr);
     }
 }
 #endif

@@ -143,13 +143,12 @@ using mbg_desc_ptr = const void*;
 
 struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 2, scan = 4, wsum_reg = 0, rate32 = 0,
-        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0;
+        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0, th_lds = 0;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
-                        sumw, compact, wsmad, sparse, prog) < std::tie(o.nw, o.block, o.discrete, o.trace, o.generic,
-                                                                       o.tables_lds, o.max_in, o.dep_batch, o.scan,
-                                                                       o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact,
-                                                                       o.wsmad, o.sparse, o.prog);
+                        sumw, compact, wsmad, sparse, prog, th_lds) <
+               std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in, o.dep_batch, o.scan,
+                        o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact, o.wsmad, o.sparse, o.prog, o.th_lds);
     }
 };
 
@@ -174,6 +173,7 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_WSMAD " << sh.wsmad << "\n";
     os << "#define MBG_SPARSE " << sh.sparse << "\n";
     os << "#define MBG_PROG " << sh.prog << "\n";
+    os << "#define MBG_TH_LDS " << sh.th_lds << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -780,11 +780,18 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     sh.sparse = sparse ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
     const Blob blob = pack_blob(cm, rt, sh.rate32 != 0, sh.compact != 0);
-    if ((size_t)lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0) >
-        (size_t)c->lds_per_block)
-        throw Error(MBG_ERR_LIMIT, "launch shape does not fit in LDS");
+    const size_t state_smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0);
+    if (state_smem > (size_t)c->lds_per_block) throw Error(MBG_ERR_LIMIT, "launch shape does not fit in LDS");
+    // the block's window-entropy partial sums go to LDS when they fit beside the shape's
+    // lanes and tables (they never change the shape itself)
+    const size_t W_run = (size_t)std::ceil(cm.max_time / cm.time_tick);
+    const size_t th_smem = W_run * 8;
+    sh.th_lds = env_int("MBG_TUNE_TH_LDS", 1) != 0 && W_run <= 4096 &&
+                        state_smem + th_smem <= (size_t)c->lds_per_block
+                    ? 1
+                    : 0;
     const Module& mod = get_module(c, cm, sh);
-    const size_t smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0);
+    const size_t smem = state_smem + (sh.th_lds ? th_smem : 0);
     int blocks_per_cu = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, mod.sim, sh.block, smem) != hipSuccess ||
         blocks_per_cu < 1)
