@@ -69,7 +69,7 @@ def embed_sources() -> str:
 def kernel_source(nw: int, block: int = 64, discrete: int = 0, trace: int = 0, generic: int = 0,
                   tables_lds: int = 1, max_in: int = 4, dep_batch: int = 2, scan: int = 4, wsum_reg: int = 0,
                   rate32: int = 0, multi: int = 0, sumw: int = 1, compact: int = 0, wsmad: int = 0,
-                  sparse: int = 0, prog: int = 0, th_lds: int = 0) -> str:
+                  sparse: int = 0, prog: int = 0, th_lds: int = 0, cells_lds: int = 0) -> str:
     with open(os.path.join(DEV, "mbg_math.h")) as f:
         math_src = f.read()
     with open(os.path.join(DEV, "mbg_kernel.hip")) as f:
@@ -80,6 +80,7 @@ def kernel_source(nw: int, block: int = 64, discrete: int = 0, trace: int = 0, g
             f"#define MBG_WSUM_REG {wsum_reg}\n#define MBG_RATE32 {rate32}\n#define MBG_MULTI {multi}\n"
             f"#define MBG_SUMW {sumw}\n#define MBG_COMPACT {compact}\n#define MBG_WSMAD {wsmad}\n"
             f"#define MBG_SPARSE {sparse}\n#define MBG_PROG {prog}\n#define MBG_TH_LDS {th_lds}\n"
+            f"#define MBG_CELLS_LDS {cells_lds}\n"
             f"#define MBG_HD __device__ __forceinline__\n{math_src}\n{kern_src}\n")
 
 
@@ -97,9 +98,10 @@ def check_kernel(verbose=False):
         for nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp, pg in shapes:
             src = os.path.join(td, f"k{nw}_{disc}_{trace}_{multi}_{cp}_{sp}_{pg}.hip")
             th = 0 if multi else 1  # single runs: window-entropy partials in LDS
+            cells = 1 if (th and not sp and nw <= 2) else 0  # small models: the whole window table too
             with open(src, "w") as f:
                 f.write(kernel_source(nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp,
-                                      pg, th))
+                                      pg, th, cells))
             cmd = [HIPCC, *KERNEL_FLAGS, "-include", "hip/hip_runtime.h", "--cuda-device-only", "-c", src, "-o", src + ".o"]
             if verbose:
                 print("+", " ".join(cmd), flush=True)

@@ -21,6 +21,7 @@
 //   MBG_SPARSE      1: more than 20 readouts, window cells and final states in a hash table
 //   MBG_PROG        1: some fast node reads more than 20 nodes (logic program, no truth table)
 //   MBG_TH_LDS      1: per-block partial sums of the window entropies in LDS (single runs)
+//   MBG_CELLS_LDS   1: also of the whole window table and final states (small tables)
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -81,9 +82,16 @@ struct mbg_node {
 #ifndef MBG_TH_LDS
 #define MBG_TH_LDS 0
 #endif
+#ifndef MBG_CELLS_LDS
+#define MBG_CELLS_LDS 0
+#endif
 #if MBG_MULTI  // sweeps: waves of one block work on different variants
 #undef MBG_TH_LDS
 #define MBG_TH_LDS 0
+#endif
+#if !MBG_TH_LDS || MBG_SPARSE  // the cell partials follow the entropy partials; dense tables only
+#undef MBG_CELLS_LDS
+#define MBG_CELLS_LDS 0
 #endif
 #define PROG_IN 255u  // in_cnt of a node whose logic is a program
 // Projected (readout) state: up to 20 bits in a dense table, up to 63 with MBG_SPARSE.
@@ -591,10 +599,16 @@ __device__ __forceinline__ void sparse_add(const mbg_desc& D, u64 key, u64 v) {
 }
 #endif
 
-__device__ __forceinline__ void flush_cell(const mbg_desc& D, int w, sidx s, u64 v) {
+// part: the block's LDS partial sums (MBG_TH_LDS): th[W], then with MBG_CELLS_LDS the window
+// table [W][2^K] and the final states [2^K].
+__device__ __forceinline__ void flush_cell(const mbg_desc& D, u64* part, int w, sidx s, u64 v) {
 #if MBG_SPARSE
+    (void)part;
     if (v) sparse_add(D, ((u64)w << D.K) | s, v);
+#elif MBG_CELLS_LDS
+    if (v) atomicAdd(part + D.W + ((u64)w << D.K) + s, v);
 #else
+    (void)part;
     if (v) atomicAdd(D.prob + ((u64)w << D.K) + s, v);
 #endif
 }
@@ -762,7 +776,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             if (w > D.W - 1) w = D.W - 1;
             if (w != cur_w) {
                 if (cur_w >= 0) {
-                    flush_cell(D, cur_w, acc_s, acc_p);
+                    flush_cell(D, thl, cur_w, acc_s, acc_p);
                     if (acc_th) flush_th(D, thl, cur_w, acc_th);
                 }
                 cur_w = w;
@@ -770,7 +784,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 acc_th = 0;
                 acc_s = s;
             } else if (s != acc_s) {
-                flush_cell(D, cur_w, acc_s, acc_p);
+                flush_cell(D, thl, cur_w, acc_s, acc_p);
                 acc_p = 0;
                 acc_s = s;
             }
@@ -788,7 +802,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 const int ww = w > D.W - 1 ? D.W - 1 : w;
                 if (ww != cur_w) {
                     if (cur_w >= 0) {
-                        flush_cell(D, cur_w, acc_s, acc_p);
+                        flush_cell(D, thl, cur_w, acc_s, acc_p);
                         if (acc_th) flush_th(D, thl, cur_w, acc_th);
                     }
                     cur_w = ww;
@@ -796,7 +810,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     acc_th = 0;
                     acc_s = s;
                 } else if (s != acc_s) {
-                    flush_cell(D, cur_w, acc_s, acc_p);
+                    flush_cell(D, thl, cur_w, acc_s, acc_p);
                     acc_p = 0;
                     acc_s = s;
                 }
@@ -812,11 +826,13 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         if (t >= D.maxt) {
             // ---- trajectory end: flush, final state, fixed point, summary ----
             if (cur_w >= 0) {
-                flush_cell(D, cur_w, acc_s, acc_p);
+                flush_cell(D, thl, cur_w, acc_s, acc_p);
                 if (acc_th) flush_th(D, thl, cur_w, acc_th);
             }
 #if MBG_SPARSE
             sparse_add(D, ((u64)D.W << D.K) | s, 1ull);
+#elif MBG_CELLS_LDS
+            atomicAdd(thl + D.W + ((u64)D.W << D.K) + s, 1ull);
 #else
             atomicAdd(D.fin + s, 1ull);
 #endif
@@ -1048,14 +1064,31 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
     u64* lane = smem;
 #endif
 #if MBG_TH_LDS
-    // the block's window-entropy partial sums follow the lanes' state
+    // the block's partial sums follow the lanes' state: th[W], then (MBG_CELLS_LDS) the
+    // window table [W][2^K] and the final states [2^K]
     u64* thl = lane + (MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
-    for (u32 i = threadIdx.x; i < (u32)D.W; i += MBG_BLOCK) thl[i] = 0;
+#if MBG_CELLS_LDS
+    const u32 S = 1u << D.K;
+    const u32 nparts = (u32)D.W + ((u32)D.W + 1u) * S;
+#else
+    const u32 nparts = (u32)D.W;
+#endif
+    for (u32 i = threadIdx.x; i < nparts; i += MBG_BLOCK) thl[i] = 0;
     __syncthreads();
     simulate_variant(D, tb, lane, thl);
     __syncthreads();  // every wave of the block has left its loop
-    for (u32 i = threadIdx.x; i < (u32)D.W; i += MBG_BLOCK)
-        if (thl[i]) atomicAdd(D.th + i, thl[i]);
+    for (u32 i = threadIdx.x; i < nparts; i += MBG_BLOCK) {
+        const u64 v = thl[i];
+        if (!v) continue;
+#if MBG_CELLS_LDS
+        const u32 c = i - (u32)D.W;  // cell index past the entropies
+        if (i < (u32)D.W) atomicAdd(D.th + i, v);
+        else if (c < (u32)D.W * S) atomicAdd(D.prob + c, v);
+        else atomicAdd(D.fin + (c - (u32)D.W * S), v);
+#else
+        atomicAdd(D.th + i, v);
+#endif
+    }
 #else
     simulate_variant(D, tb, lane, nullptr);
 #endif

@@ -143,12 +143,13 @@ using mbg_desc_ptr = const void*;
 
 struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 2, scan = 4, wsum_reg = 0, rate32 = 0,
-        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0, th_lds = 0;
+        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0, th_lds = 0, cells_lds = 0;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
-                        sumw, compact, wsmad, sparse, prog, th_lds) <
+                        sumw, compact, wsmad, sparse, prog, th_lds, cells_lds) <
                std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in, o.dep_batch, o.scan,
-                        o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact, o.wsmad, o.sparse, o.prog, o.th_lds);
+                        o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact, o.wsmad, o.sparse, o.prog, o.th_lds,
+                        o.cells_lds);
     }
 };
 
@@ -174,6 +175,7 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_SPARSE " << sh.sparse << "\n";
     os << "#define MBG_PROG " << sh.prog << "\n";
     os << "#define MBG_TH_LDS " << sh.th_lds << "\n";
+    os << "#define MBG_CELLS_LDS " << sh.cells_lds << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -782,16 +784,23 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     const Blob blob = pack_blob(cm, rt, sh.rate32 != 0, sh.compact != 0);
     const size_t state_smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0);
     if (state_smem > (size_t)c->lds_per_block) throw Error(MBG_ERR_LIMIT, "launch shape does not fit in LDS");
-    // the block's window-entropy partial sums go to LDS when they fit beside the shape's
-    // lanes and tables (they never change the shape itself)
+    // Per-block partial sums in LDS, beside the shape's lanes and tables (they never change
+    // the shape itself): the window entropies (W cells, summed by every trajectory in every
+    // window: the hottest atomics of a run), and for small dense tables the whole window
+    // table and the final states, as the reference does below 10 KB (window_average_small.cu:78).
     const size_t W_run = (size_t)std::ceil(cm.max_time / cm.time_tick);
     const size_t th_smem = W_run * 8;
     sh.th_lds = env_int("MBG_TUNE_TH_LDS", 1) != 0 && W_run <= 4096 &&
                         state_smem + th_smem <= (size_t)c->lds_per_block
                     ? 1
                     : 0;
+    const size_t cells_smem = sparse ? 0 : (W_run + 1) * ((size_t)1 << cm.K) * 8;
+    sh.cells_lds = sh.th_lds && !sparse && env_int("MBG_TUNE_CELLS_LDS", 1) != 0 && cells_smem <= (128u << 10) &&
+                           state_smem + th_smem + cells_smem <= (size_t)c->lds_per_block
+                       ? 1
+                       : 0;
     const Module& mod = get_module(c, cm, sh);
-    const size_t smem = state_smem + (sh.th_lds ? th_smem : 0);
+    const size_t smem = state_smem + (sh.th_lds ? th_smem : 0) + (sh.cells_lds ? cells_smem : 0);
     int blocks_per_cu = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, mod.sim, sh.block, smem) != hipSuccess ||
         blocks_per_cu < 1)
