@@ -97,7 +97,7 @@ def check_kernel(verbose=False):
                   (64, 128, 0, 0, 0, 0, 4, 2, 4, 2, 1, 1, 4, 0, 0, 0, 0)]
         for nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp, pg in shapes:
             src = os.path.join(td, f"k{nw}_{disc}_{trace}_{multi}_{cp}_{sp}_{pg}.hip")
-            th = 0 if multi else 1  # single runs: window-entropy partials in LDS
+            th = 1  # window-entropy partials in LDS (per block, or per wave in sweeps)
             cells = 1 if (th and not sp and nw <= 2) else 0  # small models: the whole window table too
             with open(src, "w") as f:
                 f.write(kernel_source(nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp,

@@ -20,7 +20,8 @@
 //   MBG_WSMAD       1: every rate is below 2^31 (register sum updates by multiply-add)
 //   MBG_SPARSE      1: more than 20 readouts, window cells and final states in a hash table
 //   MBG_PROG        1: some fast node reads more than 20 nodes (logic program, no truth table)
-//   MBG_TH_LDS      1: per-block partial sums of the window entropies in LDS (single runs)
+//   MBG_TH_LDS      1: partial sums of the window entropies in LDS (per block in single runs,
+//                   per wave and variant in sweeps)
 //   MBG_CELLS_LDS   1: also of the whole window table and final states (small tables)
 // and mbg_math.h.
 //
@@ -85,11 +86,8 @@ struct mbg_node {
 #ifndef MBG_CELLS_LDS
 #define MBG_CELLS_LDS 0
 #endif
-#if MBG_MULTI  // sweeps: waves of one block work on different variants
-#undef MBG_TH_LDS
-#define MBG_TH_LDS 0
-#endif
-#if !MBG_TH_LDS || MBG_SPARSE  // the cell partials follow the entropy partials; dense tables only
+#if !MBG_TH_LDS || MBG_SPARSE || MBG_MULTI  // the cell partials follow the entropy partials;
+                                           // dense tables of single runs only
 #undef MBG_CELLS_LDS
 #define MBG_CELLS_LDS 0
 #endif
@@ -1098,17 +1096,39 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
 // drains one variant at a time (its descriptor is wave-uniform, read with scalar loads),
 // starting at a variant spread by wave index and moving on when that variant's queue is
 // empty; the variants' tables stay in HBM.
-extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const mbg_desc* descs, int nv) {
+// With MBG_TH_LDS each wave sums its current variant's window entropies in its own LDS slice
+// of wmax cells (after the lanes' state) and flushes them when it moves to the next variant.
+extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const mbg_desc* descs, int nv, int wmax) {
     extern __shared__ u64 smem[];
     const u32 waves_total = gridDim.x * (MBG_BLOCK / 64);
     const u32 wave = blockIdx.x * (MBG_BLOCK / 64) + (threadIdx.x >> 6);
     const int start = (int)(((u64)wave * (u64)nv) / waves_total);
+#if MBG_TH_LDS
+    const u32 lid = threadIdx.x & 63u;
+    u64* thw = smem + (MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK + (threadIdx.x >> 6) * (u32)wmax;
+#else
+    (void)wmax;
+    u64* thw = nullptr;
+#endif
     for (int i = 0; i < nv; i++) {
         int v = start + i;
         if (v >= nv) v -= nv;
         v = __builtin_amdgcn_readfirstlane(v);
         const mbg_desc& D = descs[v];
-        simulate_variant(D, D.blob, smem, nullptr);
+#if MBG_TH_LDS
+        for (u32 c = lid; c < (u32)D.W; c += 64u) thw[c] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#endif
+        simulate_variant(D, D.blob, smem, thw);
+#if MBG_TH_LDS
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();  // every lane of the wave has left the variant's loop
+        for (u32 c = lid; c < (u32)D.W; c += 64u) {
+            const u64 x = thw[c];
+            if (x) atomicAdd(D.th + c, x);
+        }
+#endif
     }
 }
 #endif

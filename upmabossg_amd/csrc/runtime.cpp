@@ -338,6 +338,7 @@ struct SweepClass {
     DevBuf<Desc> descs;
     size_t smem = 0;
     int blocks_per_cu = 1;
+    int wmax = 1;  // largest window count of the class's variants (per-wave entropy partials)
 };
 
 struct SweepState {
@@ -1176,6 +1177,20 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             }
         }
         if (!best_waves) throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
+        // per-wave window-entropy partials in LDS when they fit without costing occupancy
+        int wmax = 1;
+        for (int v : C.members) {
+            const mbg::CompiledModel& cmv = sw->vars[v]->m->cm;
+            wmax = std::max(wmax, (int)std::ceil(cmv.max_time / cmv.time_tick));
+        }
+        const size_t state_lds = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg);
+        const size_t th_lds = (size_t)(C.sh.block / 64) * (size_t)wmax * 8;
+        C.sh.th_lds = env_int("MBG_TUNE_TH_LDS", 1) != 0 && wmax <= 4096 &&
+                              state_lds + th_lds <= (size_t)c->lds_per_block &&
+                              cu_lds / (state_lds + th_lds) == cu_lds / state_lds
+                          ? 1
+                          : 0;
+        C.wmax = wmax;
         C.sh.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 2)));
         C.sh.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
         const Module& mod = get_module(c, sw->vars[C.members[0]]->m->cm, C.sh);
@@ -1186,7 +1201,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         C.marginals = mod.marginals;
         C.marginals_batch = mod.marginals_batch;
         C.init_rates_multi = mod.init_rates_multi;
-        C.smem = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg);
+        C.smem = state_lds + (C.sh.th_lds ? th_lds : 0);
         int nb = 0;
         if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, C.sim, C.sh.block, C.smem) != hipSuccess || nb < 1) nb = 1;
         C.blocks_per_cu = nb;
@@ -1330,7 +1345,8 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (traj + C.sh.block - 1) / C.sh.block));
         const mbg_desc_ptr dp = C.descs.p;
         int n_members = (int)C.members.size();
-        void* args[] = {(void*)&dp, &n_members};
+        int wmax_arg = C.wmax;
+        void* args[] = {(void*)&dp, &n_members, &wmax_arg};
         hip_check(hipModuleLaunchKernel(C.sim, (unsigned)grid, 1, 1, (unsigned)C.sh.block, 1, 1, (unsigned)C.smem, st,
                                         args, nullptr),
                   "launch mbg_simulate_multi");
