@@ -101,6 +101,9 @@ typedef unsigned int sidx;
 #ifndef MBG_WSMAD
 #define MBG_WSMAD 0
 #endif
+#ifndef MBG_DEP_CAP
+#define MBG_DEP_CAP 0
+#endif
 
 // A node record decoded into registers (either table layout).
 struct mbg_nd {
@@ -638,11 +641,14 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
     // Step parity is kept equal across the wave: a trajectory starts only on an even loop
     // iteration (a lane refilled on an odd one holds for one iteration), so every active lane
     // has the same k & 1 and the Philox call (one per two steps) runs on every other
-    // iteration only, instead of on every iteration for a mix of parities.
+    // iteration only, instead of on every iteration for a mix of parities.  (A lane whose
+    // dependents take extra iterations steps out of phase: making it wait for its phase
+    // measured slower than the extra Philox calls.)
     u32 iter = 0;
     bool hold = false;
     u64 id = 0, total = 0, k = 0, acc_p = 0, acc_th = 0, flips = 0;
     u64 ro_R = 0, ro_S = 0;  // readout rate sum and entropy-term sum (exact)
+    u32 pend = 0, pend_off = 0;  // dependents of the last flip not yet re-evaluated
 #if MBG_WSUM_REG
     u64 ws[NSUM];
 #pragma unroll
@@ -734,6 +740,67 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         if (hold) {
             hold = false;
             continue;
+        }
+
+        // ---- re-evaluate the dependents of the last flip (the flipped node too when it reads
+        // itself or is generic), at most MBG_DEP_CAP of them per iteration (0: all): a lane with
+        // many dependents then holds the wave for fewer rounds and takes its next step later ----
+        if (pend) {
+#if MBG_DEP_CAP
+            const u32 nq = pend > (u32)MBG_DEP_CAP ? (u32)MBG_DEP_CAP : pend;
+#else
+            const u32 nq = pend;
+#endif
+            for (u32 q0 = 0; q0 < nq; q0 += DEP_BATCH) {
+                u32 kk[DEP_BATCH], xo[DEP_BATCH], xn[DEP_BATCH];
+                u64 rn[DEP_BATCH], rp[DEP_BATCH];
+                mbg_nd nk[DEP_BATCH];
+#pragma unroll
+                for (int u = 0; u < DEP_BATCH; u++) {
+                    const bool ok = q0 + u < nq;
+                    const u32 dp = TDEP(pend_off + (ok ? q0 + u : 0u));
+                    kk[u] = dp & 0xffffffu;
+                    const u32 pos = dp >> 24;
+                    nk[u] = TNODE(kk[u]);
+                    const u32 key = L.key_of(nk[u], kk[u]);
+                    xn[u] = key & 1u;
+                    xo[u] = pos == 0 ? xn[u] ^ 1u : xn[u];
+#if MBG_GENERIC
+                    if (nk[u].flags & F_GEN) {
+                        rn[u] = D.rtab[nk[u].tab_off + key];
+                        rp[u] = D.rtab[nk[u].tab_off + (key ^ (1u << pos))];
+                    } else
+#endif
+                    {
+                        const u32 logic = L.logic_of(nk[u], key);
+                        const u32 a_old = (u32)(LANE_A(kk[u] >> 6) >> (kk[u] & 63)) & 1u;
+                        rn[u] = xn[u] != logic ? TRATE(2 * kk[u] + xn[u]) : 0ull;
+                        rp[u] = a_old ? TRATE(2 * kk[u] + xo[u]) : 0ull;
+                    }
+                    if (!ok) rn[u] = rp[u];
+                }
+#pragma unroll
+                for (int u = 0; u < DEP_BATCH; u++) {
+                    if (rn[u] != rp[u]) {
+                        const u32 wk = kk[u] >> 6;
+                        const u64 bit = 1ull << (kk[u] & 63);
+                        WS_ADD(wk, rn[u] - rp[u]);
+                        total += rn[u] - rp[u];
+                        const u64 a = LANE_A(wk);
+                        LANE_A(wk) = rn[u] ? (a | bit) : (a & ~bit);
+                        if (nk[u].flags & F_RO) {
+                            const u32 ro = (nk[u].flags & F_RO) - 1u;
+                            ro_R += rn[u] - rp[u];
+                            ro_S += L.term(nk[u], ro, xn[u], rn[u]) - L.term(nk[u], ro, xo[u], rp[u]);
+                            ro_act += (int)(rn[u] != 0) - (int)(rp[u] != 0);
+                            dirty = true;
+                        }
+                    }
+                }
+            }
+            pend -= nq;
+            pend_off += nq;
+            if (pend) continue;
         }
 
         // ---- one Gillespie step (src/jit_kernels/simulation.cu:106-148) ----
@@ -987,54 +1054,9 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 dirty = true;
             }
         }
-        // re-evaluate the dependents (the flipped node too when it reads itself or is generic)
-        for (u32 q0 = 0; q0 < nj.dep_cnt; q0 += DEP_BATCH) {
-            u32 kk[DEP_BATCH], xo[DEP_BATCH], xn[DEP_BATCH];
-            u64 rn[DEP_BATCH], rp[DEP_BATCH];
-            mbg_nd nk[DEP_BATCH];
-#pragma unroll
-            for (int u = 0; u < DEP_BATCH; u++) {
-                const bool ok = q0 + u < nj.dep_cnt;
-                const u32 dp = TDEP(nj.dep_off + (ok ? q0 + u : 0u));
-                kk[u] = dp & 0xffffffu;
-                const u32 pos = dp >> 24;
-                nk[u] = TNODE(kk[u]);
-                const u32 key = L.key_of(nk[u], kk[u]);
-                xn[u] = key & 1u;
-                xo[u] = pos == 0 ? xn[u] ^ 1u : xn[u];
-#if MBG_GENERIC
-                if (nk[u].flags & F_GEN) {
-                    rn[u] = D.rtab[nk[u].tab_off + key];
-                    rp[u] = D.rtab[nk[u].tab_off + (key ^ (1u << pos))];
-                } else
-#endif
-                {
-                    const u32 logic = L.logic_of(nk[u], key);
-                    const u32 a_old = (u32)(LANE_A(kk[u] >> 6) >> (kk[u] & 63)) & 1u;
-                    rn[u] = xn[u] != logic ? TRATE(2 * kk[u] + xn[u]) : 0ull;
-                    rp[u] = a_old ? TRATE(2 * kk[u] + xo[u]) : 0ull;
-                }
-                if (!ok) rn[u] = rp[u];
-            }
-#pragma unroll
-            for (int u = 0; u < DEP_BATCH; u++) {
-                if (rn[u] != rp[u]) {
-                    const u32 wk = kk[u] >> 6;
-                    const u64 bit = 1ull << (kk[u] & 63);
-                    WS_ADD(wk, rn[u] - rp[u]);
-                    total += rn[u] - rp[u];
-                    const u64 a = LANE_A(wk);
-                    LANE_A(wk) = rn[u] ? (a | bit) : (a & ~bit);
-                    if (nk[u].flags & F_RO) {
-                        const u32 ro = (nk[u].flags & F_RO) - 1u;
-                        ro_R += rn[u] - rp[u];
-                        ro_S += L.term(nk[u], ro, xn[u], rn[u]) - L.term(nk[u], ro, xo[u], rp[u]);
-                        ro_act += (int)(rn[u] != 0) - (int)(rp[u] != 0);
-                        dirty = true;
-                    }
-                }
-            }
-        }
+        // its dependents are re-evaluated from the next iteration on
+        pend = nj.dep_cnt;
+        pend_off = nj.dep_off;
         if (nj.flags & F_RO) s ^= (sidx)1 << ((nj.flags & F_RO) - 1u);
         flips++;
 #if MBG_TRACE

@@ -143,13 +143,13 @@ using mbg_desc_ptr = const void*;
 
 struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 2, scan = 4, wsum_reg = 0, rate32 = 0,
-        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0, th_lds = 0, cells_lds = 0;
+        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0, th_lds = 0, cells_lds = 0, dep_cap = 0;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
-                        sumw, compact, wsmad, sparse, prog, th_lds, cells_lds) <
+                        sumw, compact, wsmad, sparse, prog, th_lds, cells_lds, dep_cap) <
                std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in, o.dep_batch, o.scan,
                         o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact, o.wsmad, o.sparse, o.prog, o.th_lds,
-                        o.cells_lds);
+                        o.cells_lds, o.dep_cap);
     }
 };
 
@@ -176,6 +176,7 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_PROG " << sh.prog << "\n";
     os << "#define MBG_TH_LDS " << sh.th_lds << "\n";
     os << "#define MBG_CELLS_LDS " << sh.cells_lds << "\n";
+    os << "#define MBG_DEP_CAP " << sh.dep_cap << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -464,6 +465,16 @@ int default_sumw(int nw, int wsum_reg) {
     return sw;
 }
 
+// Dependents re-evaluated per loop iteration (0: all of them).  A wave runs the dependents
+// loop as many rounds as its lane with the most dependents needs; with a cap, such a lane
+// finishes over the next iterations (and steps later) while the others go on stepping.
+// Default 4 (rounded to whole batches); MBG_TUNE_DEP_CAP=0 removes the cap.
+int default_dep_cap(int dep_batch) {
+    const int v = env_int("MBG_TUNE_DEP_CAP", std::max(4, 2 * dep_batch));
+    if (v <= 0) return 0;
+    return std::max(dep_batch, v / dep_batch * dep_batch);
+}
+
 // Launch shape: block size, table placement (LDS or HBM) and word-sum placement (LDS or
 // VGPRs).  Without tuning knobs: stage the tables in LDS when they fit beside the lanes'
 // state and keep the most waves resident per CU (160 KiB of LDS per CU on gfx950).
@@ -495,6 +506,7 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
         }
     }
     best.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 2)));
+    best.dep_cap = default_dep_cap(best.dep_batch);
     best.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
     best.sumw = default_sumw(cm.NW, best.wsum_reg);
     if (best_score < 0) {
@@ -1060,6 +1072,15 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
             sh.dep_batch = db;
             consider(sh);
         }
+        // dependents per iteration: no cap, or a few rounds of the chosen batch
+        {
+            const Shape b = best;
+            for (int rounds : {0, 2, 3, 4, 6}) {
+                Shape sh = b;
+                sh.dep_cap = rounds ? std::max(2, rounds * b.dep_batch) : 0;
+                consider(sh);
+            }
+        }
         Shape sh = lay;
         sh.scan = lay.scan == 8 ? 4 : 8;
         consider(sh);
@@ -1084,6 +1105,7 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
         out->sample = n;
         out->best_ms = best_ms;
         out->default_ms = base_ms;
+        out->dep_cap = best.dep_cap;
     }
 }
 
@@ -1192,6 +1214,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
                           : 0;
         C.wmax = wmax;
         C.sh.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 2)));
+        C.sh.dep_cap = default_dep_cap(C.sh.dep_batch);
         C.sh.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
         const Module& mod = get_module(c, sw->vars[C.members[0]]->m->cm, C.sh);
         for (int v : C.members) sw->vars[v]->mod = &mod;
