@@ -258,10 +258,13 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
 #endif
 #define LANE_S(g) ws[g]
 #if MBG_WSMAD
-// every rate change fits an i32: ws[i] += [i == g] * d as one 32 x 32 + 64 multiply-add
+// every rate change fits an i32: ws[i] += [i == g] * d as one 32 x 32 + 64 multiply-add, the
+// factor [i == g] as one bit-field extract of the one-hot word (hidden from the compiler, which
+// would otherwise rebuild each factor as a compare and a select)
 #define WS_GADD(g, d)                                                      \
     do {                                                                   \
-        const u32 oh_ = 1u << (g);                                         \
+        u32 oh_ = 1u << (g);                                               \
+        asm volatile("" : "+v"(oh_));                                      \
         const long long d_ = (long long)(int)(u32)(d);                     \
         _Pragma("unroll") for (int i_ = 0; i_ < NSUM; i_++) {              \
             u32 f_ = (oh_ >> i_) & 1u;                                     \
@@ -534,12 +537,10 @@ struct Lane {
         u32 key = xbit(i);
 #if MBG_MAX_IN <= 6
         // branch-free: reads past a node's inputs hit the next node's entries (or the
-        // table's padding) and are masked off
+        // table's padding); their bits are masked off once at the end
 #pragma unroll
-        for (u32 m = 0; m < MBG_MAX_IN; m++) {
-            const u32 b = xbit(TIN(n.in_off + m));
-            key |= (m < IN_CNT(n) ? b : 0u) << (m + 1);
-        }
+        for (u32 m = 0; m < MBG_MAX_IN; m++) key |= xbit(TIN(n.in_off + m)) << (m + 1);
+        key &= (2u << IN_CNT(n)) - 1u;
 #else
         for (u32 m = 0; m < IN_CNT(n); m++) key |= xbit(TIN(n.in_off + m)) << (m + 1);
 #endif
@@ -772,10 +773,12 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     } else
 #endif
                     {
+                        // both table reads unconditional (valid entries), then masked
                         const u32 logic = L.logic_of(nk[u], key);
                         const u32 a_old = (u32)(LANE_A(kk[u] >> 6) >> (kk[u] & 63)) & 1u;
-                        rn[u] = xn[u] != logic ? TRATE(2 * kk[u] + xn[u]) : 0ull;
-                        rp[u] = a_old ? TRATE(2 * kk[u] + xo[u]) : 0ull;
+                        const u64 r_new = TRATE(2 * kk[u] + xn[u]), r_old = TRATE(2 * kk[u] + xo[u]);
+                        rn[u] = xn[u] != logic ? r_new : 0ull;
+                        rp[u] = a_old ? r_old : 0ull;
                     }
                     if (!ok) rn[u] = rp[u];
                 }
