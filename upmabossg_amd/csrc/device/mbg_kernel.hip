@@ -813,7 +813,14 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         if (fixed_pt) {
             t1 = D.maxt;
         } else {
-            if ((k & 1ull) == 0ull) mbg_philox((u32)(k >> 1), 1u, (u32)id, D.variant_id, key0, key1, d4);
+            if ((k & 1ull) == 0ull) {
+                // the key schedule is recomputed per call on the scalar unit (the seed is
+                // wave-uniform): hoisted out of the loop, its 20 round keys would not fit in
+                // SGPRs and would come back through VALU lane reads
+                u32 k0 = __builtin_amdgcn_readfirstlane(key0), k1 = __builtin_amdgcn_readfirstlane(key1);
+                asm volatile("" : "+s"(k0), "+s"(k1));
+                mbg_philox((u32)(k >> 1), 1u, (u32)id, D.variant_id, k0, k1, d4);
+            }
             const u32 dr1 = (k & 1ull) ? d4[2] : d4[0];
             dr2 = (k & 1ull) ? d4[3] : d4[1];
 #if MBG_DISCRETE
@@ -1013,7 +1020,8 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     bb[c] = t < 64u ? (down ? 63u - t : t) : 64u;
                     mm = mm ? (mm & (mm - 1ull)) : 0ull;
                     const u32 b = bb[c] < 64u ? bb[c] : 0u;
-                    rr[c] = bb[c] < 64u ? TRATE(2 * (wbase + b) + ((xq >> b) & 1ull)) : 0ull;
+                    const u64 r_ = TRATE(2 * (wbase + b) + ((xq >> b) & 1ull));  // a valid entry
+                    rr[c] = bb[c] < 64u ? r_ : 0ull;
 #if MBG_GENERIC
                     if (bb[c] < 64u && ((gm >> b) & 1ull)) {
                         const mbg_nd nd = TNODE(wbase + b);
@@ -1023,18 +1031,18 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 }
 #pragma unroll
                 for (int c = 0; c < MBG_SCAN; c++) {
-                    if (!hit && bb[c] < 64u) {
-                        if (lim < rr[c]) {
-                            j = wbase + bb[c];
-                            r_sel = rr[c];
-                            xw = xq;
-                            hit = true;
-                        } else {
-                            lim -= rr[c];
-                        }
-                    }
+                    // branch-free: a candidate past the word's last active bit has rate 0 and
+                    // is never a hit (lim < 0 is false)
+                    const bool h = !hit && lim < rr[c];
+                    j = h ? wbase + bb[c] : j;
+                    r_sel = h ? rr[c] : r_sel;
+                    lim = hit || h ? lim : lim - rr[c];
+                    hit = hit || h;
                 }
-                if (hit) break;
+                if (hit) {
+                    xw = xq;
+                    break;
+                }
                 m = mm;
             }
         }
