@@ -67,6 +67,9 @@ struct mbg_node {
                               // dependents; bits 0-5: readout index + 1 (0: internal)
 };
 
+// Words per staged initial state (the mbg_desc::stage rows).
+#define STAGE_ROW (3 * MBG_NW + 4)
+
 #define F_GEN 0x80u
 #define F_SELF 0x40u
 #define F_RO 0x3fu
@@ -153,7 +156,9 @@ struct mbg_desc {
     const char* blob;         // model tables (layout above)
     const u64* rtab;          // fixed-point rate per key of generic nodes
     const float* istate;      // initial ON probability per node
-    u64* stage;               // [batch][3][NW]: x, act, wsum of each trajectory's initial state
+    u64* stage;               // [batch][STAGE_ROW]: x, act, wsum (NW words each) of each trajectory's
+                              // initial state, then its projected state, readout rate sum,
+                              // readout entropy-term sum and active-readout count
     u64* prob;                // [W][2^K] fixed-point sojourn time (counts when discrete)
     u64* th;                  // [W] fixed-point transition entropy * time
     u64* fin;                 // [2^K] final projected states
@@ -327,7 +332,7 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_state(mbg_desc D) {
     if (gid >= D.batch * MBG_NW) return;
     const u64 t = gid / MBG_NW;
     const int w = (int)(gid - t * MBG_NW);
-    D.stage[t * 3 * MBG_NW + w] = initial_word(D, (u32)(D.batch_first + t), w);
+    D.stage[t * STAGE_ROW + w] = initial_word(D, (u32)(D.batch_first + t), w);
 }
 
 __device__ __forceinline__ u64 rate_from_key(const mbg_desc& D, const char* tb, const mbg_nd& n, u32 i, u32 key) {
@@ -369,7 +374,7 @@ __device__ __forceinline__ void init_rates_block(const mbg_desc& D, u64 t0) {
         u64 word = 0;
         if (t0 + l < D.batch) {
             word = initial_word(D, (u32)(D.batch_first + t0 + l), (int)w);
-            D.stage[(t0 + l) * 3 * MBG_NW + w] = word;
+            D.stage[(t0 + l) * STAGE_ROW + w] = word;
         }
         xs[idx] = word;
     }
@@ -474,8 +479,8 @@ __device__ __forceinline__ void init_rates_block(const mbg_desc& D, u64 t0) {
             }
         }
         if (t < D.batch) {
-            D.stage[t * 3 * MBG_NW + MBG_NW + w] = a;
-            D.stage[t * 3 * MBG_NW + 2 * MBG_NW + w] = sum;
+            D.stage[t * STAGE_ROW + MBG_NW + w] = a;
+            D.stage[t * STAGE_ROW + 2 * MBG_NW + w] = sum;
         }
     }
 }
@@ -500,7 +505,7 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
     if (gid >= D.batch * MBG_NW) return;
     const u64 t = gid / MBG_NW;
     const int w = (int)(gid - t * MBG_NW);
-    const u64* x = D.stage + t * 3 * MBG_NW;
+    const u64* x = D.stage + t * STAGE_ROW;
     const char* tb = D.blob;
     u64 a = 0, sum = 0;
     for (int b = 0; b < 64; b++) {
@@ -518,8 +523,75 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
             sum += r;
         }
     }
-    D.stage[t * 3 * MBG_NW + MBG_NW + w] = a;
-    D.stage[t * 3 * MBG_NW + 2 * MBG_NW + w] = sum;
+    D.stage[t * STAGE_ROW + MBG_NW + w] = a;
+    D.stage[t * STAGE_ROW + 2 * MBG_NW + w] = sum;
+}
+#endif
+
+// Entropy term rint(r log2(r) 2^sigma) of readout ro (index) at rate r produced with own state
+// bit xi: a table entry for fast nodes, computed for generic ones.
+__device__ __forceinline__ u64 entropy_term(const mbg_desc& D, const char* tb, const mbg_nd& n, u32 ro, u32 xi,
+                                            u64 r) {
+    if (!r) return 0ull;
+#if MBG_GENERIC
+    if (n.flags & F_GEN) {
+        const double rd = (double)r;
+        return (u64)rint(ldexp(rd * mbg_log2(rd), D.sigma));
+    }
+#else
+    (void)D;
+    (void)n;
+#endif
+    return TTERM(2 * ro + xi);
+}
+
+// Pass 3, one thread per staged trajectory: the readout part of its initial state (projected
+// state, readout rate sum, entropy-term sum, active-readout count) at the end of its stage row,
+// so that a lane starting the trajectory reads four words instead of looping over the readouts
+// while the rest of its wave waits.
+__device__ __forceinline__ void init_readouts_one(const mbg_desc& D, u64 t) {
+    u64* row = D.stage + t * STAGE_ROW;
+    const char* tb = D.blob;
+    u64 s = 0, R = 0, S = 0, act = 0;
+    for (int j = 0; j < D.K; j++) {
+        const u32 n = TRO(j);
+        const u32 xi = (u32)(row[n >> 6] >> (n & 63)) & 1u;
+        s |= (u64)xi << j;
+        const mbg_nd nd = TNODE(n);
+        u64 r;
+#if MBG_GENERIC
+        if (nd.flags & F_GEN) {
+            u32 key = xi;
+            for (u32 m = 0; m < IN_CNT(nd); m++) {
+                const u32 q = TIN(nd.in_off + m);
+                key |= ((u32)(row[q >> 6] >> (q & 63)) & 1u) << (m + 1);
+            }
+            r = rate_from_key(D, tb, nd, n, key);
+        } else
+#endif
+            r = ((row[MBG_NW + (n >> 6)] >> (n & 63)) & 1ull) ? TRATE(2 * n + xi) : 0ull;
+        R += r;
+        S += entropy_term(D, tb, nd, (u32)j, xi, r);
+        act += r != 0 ? 1ull : 0ull;
+    }
+    row[3 * MBG_NW + 0] = s;
+    row[3 * MBG_NW + 1] = R;
+    row[3 * MBG_NW + 2] = S;
+    row[3 * MBG_NW + 3] = act;
+}
+
+extern "C" __global__ __launch_bounds__(256) void mbg_init_readouts(mbg_desc D) {
+    const u64 t = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (t < D.batch) init_readouts_one(D, t);
+}
+
+#if MBG_MULTI
+// The same for every variant of a sweep class: block (x, v) takes trajectories 256x ..
+// 256x + 255 of variant v.
+extern "C" __global__ __launch_bounds__(256) void mbg_init_readouts_multi(const mbg_desc* descs) {
+    const mbg_desc& D = descs[blockIdx.y];
+    const u64 t = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (t < D.batch) init_readouts_one(D, t);
 }
 #endif
 
@@ -556,17 +628,8 @@ struct Lane {
         return (TLUT(bp >> 5) >> (bp & 31)) & 1u;
     }
 
-    // Entropy term rint(r log2(r) 2^sigma) of readout ro (index) at rate r produced with
-    // own state bit xi: a table entry for fast nodes, computed for generic ones.
     __device__ __forceinline__ u64 term(const mbg_nd& n, u32 ro, u32 xi, u64 r) const {
-        if (!r) return 0ull;
-#if MBG_GENERIC
-        if (n.flags & F_GEN) {
-            const double rd = (double)r;
-            return (u64)rint(ldexp(rd * mbg_log2(rd), D.sigma));
-        }
-#endif
-        return TTERM(2 * ro + xi);
+        return entropy_term(D, tb, n, ro, xi, r);
     }
 };
 
@@ -682,7 +745,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     have = true;
                     hold = odd_iter;
                     id = D.batch_first + g;
-                    const u64* st = D.stage + g * 3 * MBG_NW;
+                    const u64* st = D.stage + g * STAGE_ROW;
                     total = 0;
 #pragma unroll
                     for (int g = 0; g < NSUM; g++) {
@@ -701,25 +764,10 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                         LANE_S(g) = gs;
                         total += gs;
                     }
-                    s = 0;
-                    ro_R = 0;
-                    ro_S = 0;
-                    ro_act = 0;
-                    for (int j = 0; j < D.K; j++) {
-                        const u32 n = TRO(j);
-                        const u32 xi = L.xbit(n);
-                        s |= (sidx)xi << j;
-                        const mbg_nd nd = TNODE(n);
-                        u64 r;
-#if MBG_GENERIC
-                        if (nd.flags & F_GEN) r = rate_from_key(D, tb, nd, n, L.key_of(nd, n));
-                        else
-#endif
-                            r = ((LANE_A(n >> 6) >> (n & 63)) & 1ull) ? TRATE(2 * n + xi) : 0ull;
-                        ro_R += r;
-                        ro_S += L.term(nd, (u32)j, xi, r);
-                        ro_act += r != 0;
-                    }
+                    s = (sidx)st[3 * MBG_NW + 0];
+                    ro_R = st[3 * MBG_NW + 1];
+                    ro_S = st[3 * MBG_NW + 2];
+                    ro_act = (int)st[3 * MBG_NW + 3];
                     t = 0.0;
                     k = 0;
                     cur_w = -1;

@@ -270,8 +270,9 @@ Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt, bool rat
 
 struct Module {
     hipModule_t mod = nullptr;
-    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr,
-                  marginals_batch = nullptr, init_rates_multi = nullptr, fp_dedup = nullptr, fp_compact = nullptr,
+    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, init_readouts = nullptr,
+                  init_readouts_multi = nullptr, marginals = nullptr, marginals_batch = nullptr,
+                  init_rates_multi = nullptr, fp_dedup = nullptr, fp_compact = nullptr,
                   fp_gather = nullptr, cells_count = nullptr, cells_compact = nullptr;
 };
 
@@ -333,8 +334,9 @@ struct MargJob {
 
 struct SweepClass {
     Shape sh{};
-    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, marginals = nullptr,
-                  marginals_batch = nullptr, init_rates_multi = nullptr;
+    hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, init_readouts = nullptr,
+                  init_readouts_multi = nullptr, marginals = nullptr, marginals_batch = nullptr,
+                  init_rates_multi = nullptr;
     std::vector<int> members;
     DevBuf<Desc> descs;
     size_t smem = 0;
@@ -429,8 +431,16 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
         hip_check(hipModuleGetFunction(&m.init_rates_multi, m.mod, "mbg_init_rates_multi"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_state, m.mod, "mbg_init_state"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_rates, m.mod, "mbg_init_rates"), "hipModuleGetFunction");
+    hip_check(hipModuleGetFunction(&m.init_readouts, m.mod, "mbg_init_readouts"), "hipModuleGetFunction");
+    if (sh.multi)
+        hip_check(hipModuleGetFunction(&m.init_readouts_multi, m.mod, "mbg_init_readouts_multi"),
+                  "hipModuleGetFunction");
     return c->modules.emplace(sh, m).first->second;
 }
+
+// 64-bit words of a staged initial state (x, act and word sums, then the readout part: projected
+// state, readout rate sum, entropy-term sum, active-readout count), as STAGE_ROW in the kernel.
+uint64_t stage_row(int nw) { return 3ull * (uint64_t)nw + 4ull; }
 
 // LDS of the lanes' state: x and act, plus the word rate sums unless they are in VGPRs only
 size_t lane_lds_bytes(int nw, int block, int wsum_reg) { return (size_t)(wsum_reg == 1 ? 2 : 3) * nw * block * 8; }
@@ -835,7 +845,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     // Fixed points: one slot per simulated trajectory, within a 16 GiB budget.
     const uint64_t fp_cap = std::min<uint64_t>(count, (16ull << 30) / (8ull * (uint64_t)cm.NW));
     // Initial states are staged per batch, within a 8 GiB budget.
-    const uint64_t batch_max = std::max<uint64_t>(1, std::min<uint64_t>(count, (8ull << 30) / (24ull * (uint64_t)cm.NW)));
+    const uint64_t batch_max = std::max<uint64_t>(1, std::min<uint64_t>(count, (8ull << 30) / (8ull * stage_row(cm.NW))));
 
     hipStream_t s = c->stream;
     RunBuffers& B = c->buf;
@@ -854,7 +864,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     d_blob.upload(blob.words, s);
     d_rtab.upload(rt.rtab, s);
     d_istate.upload(rt.istate, s);
-    d_stage.alloc((size_t)batch_max * 3 * cm.NW);
+    d_stage.alloc((size_t)batch_max * stage_row(cm.NW));
     d_fp.alloc((size_t)fp_cap * cm.NW);
     const uint64_t summ_cap = p->trace ? count : 0;
     d_work.alloc(1);
@@ -930,6 +940,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
             const uint64_t rate_blocks = cm.NW <= 128 ? (D.batch + 63) / 64 : (init_threads + 255) / 256;
             if (cm.NW > 128) launch(mod.init_state, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_state");
             launch(mod.init_rates, rate_blocks, 256, 0, s, D, "launch mbg_init_rates");
+            launch(mod.init_readouts, (D.batch + 255) / 256, 256, 0, s, D, "launch mbg_init_readouts");
             uint64_t grid = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)blocks_per_cu;
             grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (D.batch + sh.block - 1) / sh.block));
             hip_check(hipEventRecord(e0, s), "hipEventRecord");
@@ -1221,6 +1232,8 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         C.sim = mod.sim;
         C.init_state = mod.init_state;
         C.init_rates = mod.init_rates;
+        C.init_readouts = mod.init_readouts;
+        C.init_readouts_multi = mod.init_readouts_multi;
         C.marginals = mod.marginals;
         C.marginals_batch = mod.marginals_batch;
         C.init_rates_multi = mod.init_rates_multi;
@@ -1251,7 +1264,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             B.blob.upload(blob.words, main);
             B.rtab.upload(rt.rtab, main);
             B.istate.upload(rt.istate, main);
-            B.stage.alloc((size_t)V.n * 3 * V.NWc);
+            B.stage.alloc((size_t)V.n * stage_row(V.NWc));
             B.prob.zero((size_t)rs.W * rs.S, main);
             B.th.zero((size_t)rs.W, main);
             B.fin.zero((size_t)rs.S, main);
@@ -1353,6 +1366,11 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             hip_check(hipModuleLaunchKernel(C.init_rates_multi, (unsigned)bx, (unsigned)C.members.size(), 1, 256, 1, 1, 0,
                                             st, iargs, nullptr),
                       "launch mbg_init_rates_multi");
+            uint64_t rx = 1;
+            for (int v : C.members) rx = std::max<uint64_t>(rx, (sw->vars[v]->D.batch + 255) / 256);
+            hip_check(hipModuleLaunchKernel(C.init_readouts_multi, (unsigned)rx, (unsigned)C.members.size(), 1, 256, 1,
+                                            1, 0, st, iargs, nullptr),
+                      "launch mbg_init_readouts_multi");
         } else {
             for (int v : C.members) {
                 Desc& D = sw->vars[v]->D;
@@ -1360,6 +1378,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
                 const uint64_t init_threads = D.batch * nwc;
                 launch(C.init_state, (init_threads + 255) / 256, 256, 0, st, D, "launch mbg_init_state");
                 launch(C.init_rates, (init_threads + 255) / 256, 256, 0, st, D, "launch mbg_init_rates");
+                launch(C.init_readouts, (D.batch + 255) / 256, 256, 0, st, D, "launch mbg_init_readouts");
             }
         }
         uint64_t traj = 0;
