@@ -1058,20 +1058,23 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             const u64 gm = TGEN(wi);
 #endif
             const u32 wbase = wi * 64u;
+            const u32 flip = down ? 63u : 0u;  // bit-reversed order: candidate t is bit 63 - t
             while (m) {
-                // MBG_SCAN candidates per round trip to the rate table
+                // MBG_SCAN candidates per round trip to the rate table; past the last active
+                // bit (t = 64) a candidate still reads a valid entry (bit 0 or 63 of the word)
+                // and its rate is masked to 0
                 u32 bb[MBG_SCAN];
                 u64 rr[MBG_SCAN], mm = m;
 #pragma unroll
                 for (int c = 0; c < MBG_SCAN; c++) {
                     const u32 t = mm ? (u32)__builtin_ctzll(mm) : 64u;
-                    bb[c] = t < 64u ? (down ? 63u - t : t) : 64u;
-                    mm = mm ? (mm & (mm - 1ull)) : 0ull;
-                    const u32 b = bb[c] < 64u ? bb[c] : 0u;
-                    const u64 r_ = TRATE(2 * (wbase + b) + ((xq >> b) & 1ull));  // a valid entry
-                    rr[c] = bb[c] < 64u ? r_ : 0ull;
+                    const u32 b = (t ^ flip) & 63u;
+                    bb[c] = b;
+                    mm &= mm - 1ull;
+                    const u64 r_ = TRATE(2 * (wbase + b) + ((xq >> b) & 1ull));
+                    rr[c] = t < 64u ? r_ : 0ull;
 #if MBG_GENERIC
-                    if (bb[c] < 64u && ((gm >> b) & 1ull)) {
+                    if (t < 64u && ((gm >> b) & 1ull)) {
                         const mbg_nd nd = TNODE(wbase + b);
                         rr[c] = rate_from_key(D, tb, nd, wbase + b, L.key_of(nd, wbase + b));
                     }
@@ -1080,7 +1083,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #pragma unroll
                 for (int c = 0; c < MBG_SCAN; c++) {
                     // branch-free: a candidate past the word's last active bit has rate 0 and
-                    // is never a hit (lim < 0 is false)
+                    // is never a hit (lim < 0 is false), so a hit's bb[c] is a real candidate
                     const bool h = !hit && lim < rr[c];
                     j = h ? wbase + bb[c] : j;
                     r_sel = h ? rr[c] : r_sel;

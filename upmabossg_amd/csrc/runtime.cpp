@@ -478,11 +478,11 @@ int default_sumw(int nw, int wsum_reg) {
 // Dependents re-evaluated per loop iteration (0: all of them).  A wave runs the dependents
 // loop as many rounds as its lane with the most dependents needs; with a cap, such a lane
 // finishes over the next iterations (and steps later) while the others go on stepping.
-// Default 4 (rounded to whole batches); MBG_TUNE_DEP_CAP=0 removes the cap.
+// Default max(4, two batches); MBG_TUNE_DEP_CAP sets it (0 removes the cap).  The last round
+// of a capped iteration may be a partial batch.
 int default_dep_cap(int dep_batch) {
     const int v = env_int("MBG_TUNE_DEP_CAP", std::max(4, 2 * dep_batch));
-    if (v <= 0) return 0;
-    return std::max(dep_batch, v / dep_batch * dep_batch);
+    return v <= 0 ? 0 : v;
 }
 
 // Launch shape: block size, table placement (LDS or HBM) and word-sum placement (LDS or
