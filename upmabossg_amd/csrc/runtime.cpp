@@ -478,10 +478,17 @@ int default_sumw(int nw, int wsum_reg) {
 // Dependents re-evaluated per loop iteration (0: all of them).  A wave runs the dependents
 // loop as many rounds as its lane with the most dependents needs; with a cap, such a lane
 // finishes over the next iterations (and steps later) while the others go on stepping.
-// Default max(4, two batches); MBG_TUNE_DEP_CAP sets it (0 removes the cap).  The last round
-// of a capped iteration may be a partial batch.
-int default_dep_cap(int dep_batch) {
-    const int v = env_int("MBG_TUNE_DEP_CAP", std::max(4, 2 * dep_batch));
+// The cap pays where a few nodes have many more dependents than the mean (random networks:
+// roughly Poisson out-degrees); where every node has about as many (the ring models), it only
+// splits each flip's work over more iterations.  Default: 1.5 times the mean dependent count,
+// at least 4 and at least two batches; MBG_TUNE_DEP_CAP sets it (0 removes the cap).  The last
+// round of a capped iteration may be a partial batch.
+int default_dep_cap(const mbg::CompiledModel& cm, int dep_batch) {
+    double sum = 0;
+    for (const mbg::NodeRec& n : cm.nodes) sum += n.dep_cnt;
+    const double mean = cm.N ? sum / cm.N : 0.0;
+    const int dflt = std::max({4, 2 * dep_batch, (int)std::ceil(1.5 * mean)});
+    const int v = env_int("MBG_TUNE_DEP_CAP", dflt);
     return v <= 0 ? 0 : v;
 }
 
@@ -516,7 +523,7 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
         }
     }
     best.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 2)));
-    best.dep_cap = default_dep_cap(best.dep_batch);
+    best.dep_cap = default_dep_cap(cm, best.dep_batch);
     best.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
     best.sumw = default_sumw(cm.NW, best.wsum_reg);
     if (best_score < 0) {
@@ -1225,7 +1232,9 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
                           : 0;
         C.wmax = wmax;
         C.sh.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 2)));
-        C.sh.dep_cap = default_dep_cap(C.sh.dep_batch);
+        C.sh.dep_cap = 0;
+        for (int v : C.members)
+            C.sh.dep_cap = std::max(C.sh.dep_cap, default_dep_cap(sw->vars[v]->m->cm, C.sh.dep_batch));
         C.sh.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
         const Module& mod = get_module(c, sw->vars[C.members[0]]->m->cm, C.sh);
         for (int v : C.members) sw->vars[v]->mod = &mod;
