@@ -137,6 +137,21 @@ def test_fixed_points_deduplicated_on_device_bit_exact(gpu_ctx, n):
     assert g.fixed_point_trajectories > 4096 and len(g.fixed_counts) == len(r.fixed_counts)
 
 
+@pytest.mark.parametrize("cap", ["1", "2", "0"])
+def test_dependents_cap_bit_exact(gpu_ctx, monkeypatch, cap):
+    """A cap on the dependents re-evaluated per loop iteration only reschedules the work (a lane
+    with more finishes them in later iterations before its next step): with every dependent
+    deferred (1), two per iteration, or no cap, the results are the oracle's.  The cap is read
+    when a launch shape is built, so each value compiles its own kernel."""
+    monkeypatch.setenv("MBG_TUNE_DEP_CAP", cap)
+    monkeypatch.setenv("MBG_TUNE_DEP_BATCH", "1")
+    net = synth.random_network(300, 1, 6, 5, rates="loguniform", istate="uniform", readouts=12, special_frac=0.2,
+                               max_time=20.0, time_tick=0.5, sample_count=3000)
+    _compare(gpu_ctx, net.bnd(), net.cfg(), count=3000)
+    b, c = model_text("sizek")
+    _compare(gpu_ctx, b, c, count=1000)
+
+
 def test_block_size_and_grid_independent(gpu_ctx):
     net = synth.config_small()
     m = Model(net.bnd(), net.cfg())
