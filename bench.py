@@ -275,7 +275,7 @@ def main():
             "kernel_ms": kern_s * 1e3,
             "transitions_per_s_with_host_copy": transitions / wall_host,
             "launch_shape": ({k: tune[k] for k in ("block", "tables_in_lds", "wsum_reg", "compact", "dep_batch",
-                                                   "dep_cap", "scan", "wsmad")} if tune else "untuned"),
+                                                   "dep_cap", "scan", "wsmad", "state_hbm")} if tune else "untuned"),
         },
         "roofline": {
             "bound": "valu",

@@ -114,8 +114,8 @@ mbg_status mbg_run(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p);
 /* Measured launch shape (no reference counterpart; the reference compiles one fixed launch
  * configuration): times candidate shapes on `sample` trajectories of the model (0: up to
  * 262,144), by coordinate descent over table placement, compact tables, register or LDS rate
- * sums, block size, dependents per round and per iteration, search width and multiply-add sum
- * updates, and
+ * sums (or the lanes' whole state in HBM), block size, dependents per round and per iteration,
+ * search width and multiply-add sum updates, and
  * keeps the fastest for later runs of models of the same structure (explicit blocks in
  * mbg_run_params override it).  Results never depend on the shape.  Discards the last run. */
 typedef struct mbg_tune_info {
@@ -123,6 +123,7 @@ typedef struct mbg_tune_info {
     uint64_t sample;
     double best_ms, default_ms;   /* kernel time of the chosen and of the untuned shape */
     int dep_cap;                  /* dependents re-evaluated per loop iteration (0: all) */
+    int state_hbm;                /* 1: the lanes' state in HBM instead of LDS */
 } mbg_tune_info;
 mbg_status mbg_autotune(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, mbg_tune_info* out);
 

@@ -152,6 +152,33 @@ def test_dependents_cap_bit_exact(gpu_ctx, monkeypatch, cap):
     _compare(gpu_ctx, b, c, count=1000)
 
 
+@pytest.mark.parametrize("case", ["random", "sparse", "ring", "discrete", "wide_state"])
+def test_lane_state_in_hbm_bit_exact(gpu_ctx, monkeypatch, case):
+    """The lanes' state (x, act, word sums) in an HBM buffer instead of LDS (MBG_STATE_HBM, the
+    default above one wave's worth of a CU's LDS) is a layout only: forced on for models that
+    would keep it in LDS (register word sums, a sparse window table, logic programs, discrete
+    time) and left to the default on a 79-word model, the results are the oracle's."""
+    monkeypatch.setenv("MBG_TUNE_STATE_HBM", "0" if case == "wide_state" else "1")
+    if case == "random":
+        net = synth.random_network(300, 1, 6, 7, rates="loguniform", istate="uniform", readouts=12,
+                                   special_frac=0.2, max_time=20.0, time_tick=0.5, sample_count=3000)
+        _compare(gpu_ctx, net.bnd(), net.cfg(), count=3000)
+    elif case == "sparse":
+        net = synth.random_network(120, 2, 3, 9, rates="loguniform", istate="uniform", readouts=30,
+                                   max_time=10.0, time_tick=0.5, sample_count=2000)
+        _compare(gpu_ctx, net.bnd(), net.cfg(), count=2000)
+    elif case == "ring":
+        b, c = synth.ring_network(100, steps_to_fixed_point=50, sample_count=2000, time_tick=5.0, max_time=100.0)
+        _compare(gpu_ctx, b, c, count=2000)
+    elif case == "discrete":
+        b, c = model_text("cellcycle")
+        _compare(gpu_ctx, b, c, count=5000)
+    else:
+        monkeypatch.delenv("MBG_TUNE_STATE_HBM")
+        net = synth.config_fixed_heavy()
+        _compare(gpu_ctx, net.bnd(), net.cfg(), count=512, n=262144)
+
+
 def test_block_size_and_grid_independent(gpu_ctx):
     net = synth.config_small()
     m = Model(net.bnd(), net.cfg())

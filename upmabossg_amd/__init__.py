@@ -42,7 +42,8 @@ class MbgError(RuntimeError):
 class _TuneInfo(C.Structure):
     _fields_ = [("block", C.c_int), ("tables_in_lds", C.c_int), ("wsum_reg", C.c_int), ("compact", C.c_int),
                 ("dep_batch", C.c_int), ("scan", C.c_int), ("wsmad", C.c_int), ("trials", C.c_int),
-                ("sample", C.c_uint64), ("best_ms", C.c_double), ("default_ms", C.c_double), ("dep_cap", C.c_int)]
+                ("sample", C.c_uint64), ("best_ms", C.c_double), ("default_ms", C.c_double), ("dep_cap", C.c_int),
+                ("state_hbm", C.c_int)]
 
 
 class _RunParams(C.Structure):

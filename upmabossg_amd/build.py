@@ -69,7 +69,8 @@ def embed_sources() -> str:
 def kernel_source(nw: int, block: int = 64, discrete: int = 0, trace: int = 0, generic: int = 0,
                   tables_lds: int = 1, max_in: int = 4, dep_batch: int = 2, scan: int = 4, wsum_reg: int = 0,
                   rate32: int = 0, multi: int = 0, sumw: int = 1, compact: int = 0, wsmad: int = 0,
-                  sparse: int = 0, prog: int = 0, th_lds: int = 0, cells_lds: int = 0, dep_cap: int = 0) -> str:
+                  sparse: int = 0, prog: int = 0, th_lds: int = 0, cells_lds: int = 0, dep_cap: int = 0,
+                  state_hbm: int = 0) -> str:
     with open(os.path.join(DEV, "mbg_math.h")) as f:
         math_src = f.read()
     with open(os.path.join(DEV, "mbg_kernel.hip")) as f:
@@ -80,7 +81,7 @@ def kernel_source(nw: int, block: int = 64, discrete: int = 0, trace: int = 0, g
             f"#define MBG_WSUM_REG {wsum_reg}\n#define MBG_RATE32 {rate32}\n#define MBG_MULTI {multi}\n"
             f"#define MBG_SUMW {sumw}\n#define MBG_COMPACT {compact}\n#define MBG_WSMAD {wsmad}\n"
             f"#define MBG_SPARSE {sparse}\n#define MBG_PROG {prog}\n#define MBG_TH_LDS {th_lds}\n"
-            f"#define MBG_CELLS_LDS {cells_lds}\n#define MBG_DEP_CAP {dep_cap}\n"
+            f"#define MBG_CELLS_LDS {cells_lds}\n#define MBG_DEP_CAP {dep_cap}\n#define MBG_STATE_HBM {state_hbm}\n"
             f"#define MBG_HD __device__ __forceinline__\n{math_src}\n{kern_src}\n")
 
 
@@ -100,9 +101,10 @@ def check_kernel(verbose=False):
             th = 1  # window-entropy partials in LDS (per block, or per wave in sweeps)
             cells = 1 if (th and not sp and nw <= 2) else 0  # small models: the whole window table too
             cap = 2 * db if i % 2 else 0  # dependents per iteration capped in every other shape
+            hbm = 1 if (not multi and i in (3, 5)) else 0  # lanes' state in HBM
             with open(src, "w") as f:
                 f.write(kernel_source(nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp,
-                                      pg, th, cells, cap))
+                                      pg, th, cells, cap, hbm))
             cmd = [HIPCC, *KERNEL_FLAGS, "-include", "hip/hip_runtime.h", "--cuda-device-only", "-c", src, "-o", src + ".o"]
             if verbose:
                 print("+", " ".join(cmd), flush=True)

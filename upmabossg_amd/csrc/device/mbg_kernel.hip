@@ -23,6 +23,9 @@
 //   MBG_TH_LDS      1: partial sums of the window entropies in LDS (per block in single runs,
 //                   per wave and variant in sweeps)
 //   MBG_CELLS_LDS   1: also of the whole window table and final states (small tables)
+//   MBG_STATE_HBM   1: the lanes' x, act and word sums in an HBM buffer (lane-interleaved over
+//                   the whole grid) instead of LDS, for models whose per-lane state would leave
+//                   a CU with one or two waves (single runs only)
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -107,6 +110,12 @@ typedef unsigned int sidx;
 #ifndef MBG_DEP_CAP
 #define MBG_DEP_CAP 0
 #endif
+#ifndef MBG_STATE_HBM
+#define MBG_STATE_HBM 0
+#endif
+#if MBG_STATE_HBM && MBG_MULTI
+#error "the lane state in HBM is a layout of single runs"
+#endif
 
 // A node record decoded into registers (either table layout).
 struct mbg_nd {
@@ -186,6 +195,11 @@ struct mbg_desc {
     u64* cell_vals;           // [cell_mask + 1] fixed-point sums / counts
     u64* cell_miss;           // inserts that found no free slot within the probe limit
     u64 cell_mask;
+    // MBG_STATE_HBM: the lanes' state, element (array, word) of lane t at
+    // (array * NW + word) * lane_stride + t, for t below lane_stride (at least the grid's threads)
+    u64* lane_state;
+    u32 lane_stride;
+    u32 pad2_;
 };
 
 #if MBG_COMPACT
@@ -251,8 +265,15 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
 #define TTERM(i) (((const u64*)(tb + D.lay.terms))[i])
 #define TGEN(i) (((const u64*)(tb + D.lay.gen_mask))[i])
 
-#define LANE_X(w) lane[(0 * MBG_NW + (w)) * MBG_BLOCK + tid]
-#define LANE_A(w) lane[(1 * MBG_NW + (w)) * MBG_BLOCK + tid]
+// Lane state: element w of a lane's array at w * LSTRIDE + tid (no two lanes of a wave share
+// an LDS bank whatever word each touches; in HBM, consecutive lanes share cache lines).
+#if MBG_STATE_HBM
+#define LSTRIDE D.lane_stride
+#else
+#define LSTRIDE MBG_BLOCK
+#endif
+#define LANE_X(w) lane[(0 * MBG_NW + (w)) * LSTRIDE + tid]
+#define LANE_A(w) lane[(1 * MBG_NW + (w)) * LSTRIDE + tid]
 #if MBG_WSUM_REG
 // rate sums of groups of MBG_SUMW words in registers: every access is an unrolled select
 // over the groups (at most 16)
@@ -287,7 +308,7 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
 #endif
 #if MBG_WSUM_REG == 2
 // the word sums themselves in LDS (lane-interleaved like x and act)
-#define LANE_W(w) lane[(2 * MBG_NW + (w)) * MBG_BLOCK + tid]
+#define LANE_W(w) lane[(2 * MBG_NW + (w)) * LSTRIDE + tid]
 #define WS_ADD(w, d)                                                       \
     do {                                                                   \
         const u32 w2_ = (w);                                               \
@@ -302,7 +323,7 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
 #undef MBG_SUMW
 #define MBG_SUMW 1
 #define NSUM MBG_NW
-#define LANE_S(w) lane[(2 * MBG_NW + (w)) * MBG_BLOCK + tid]
+#define LANE_S(w) lane[(2 * MBG_NW + (w)) * LSTRIDE + tid]
 #define WS_ADD(w, d) LANE_S(w) = LANE_S(w) + (d)
 #endif
 
@@ -696,8 +717,9 @@ __device__ __forceinline__ void flush_th(const mbg_desc& D, u64* thl, int w, u64
 }
 
 __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* tb, u64* lane, u64* thl) {
-    const u32 tid = threadIdx.x;
-    const u32 lid = tid & 63u;
+    // index of this lane's state: its thread in the block (LDS) or in the grid (HBM)
+    const u32 tid = MBG_STATE_HBM ? blockIdx.x * MBG_BLOCK + threadIdx.x : threadIdx.x;
+    const u32 lid = threadIdx.x & 63u;
     Lane L{D, tb, lane, tid};
     const u32 key0 = (u32)D.seed, key1 = (u32)(D.seed >> 32);
 
@@ -750,7 +772,13 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #pragma unroll
                     for (int g = 0; g < NSUM; g++) {
                         u64 gs = 0;
+                        // groups of several words are copied word by word, so that the copies
+                        // of a wide state do not all sit in registers at once
+#if MBG_SUMW > 2
+#pragma unroll 1
+#else
 #pragma unroll
+#endif
                         for (int q = 0; q < MBG_SUMW; q++) {
                             const int w = g * MBG_SUMW + q;
                             if (MBG_WSUM_REG == 2 && w >= MBG_NW) continue;  // short last group
@@ -962,8 +990,10 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             atomicAdd(D.flip_total, flips);
             if (fixed_pt) {
                 const u64 slot = atomicAdd(D.fp_count, 1ull);
-                if (slot < D.fp_cap)
+                if (slot < D.fp_cap) {
+#pragma unroll 4
                     for (int w = 0; w < MBG_NW; w++) D.fp_states[slot * MBG_NW + w] = LANE_X(w);
+                }
             }
 #if MBG_TRACE
             const u64 loc = id - D.first_id;
@@ -1140,15 +1170,22 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
         __syncthreads();
     }
     const char* tb = (const char*)smem;
-    u64* lane = smem + D.lay.bytes / 8;
+    u64* after = smem + D.lay.bytes / 8;
 #else
     const char* tb = D.blob;
-    u64* lane = smem;
+    u64* after = smem;
+#endif
+#if MBG_STATE_HBM
+    u64* lane = D.lane_state;
+    u64* part = after;
+#else
+    u64* lane = after;
+    u64* part = lane + (MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
 #endif
 #if MBG_TH_LDS
-    // the block's partial sums follow the lanes' state: th[W], then (MBG_CELLS_LDS) the
-    // window table [W][2^K] and the final states [2^K]
-    u64* thl = lane + (MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
+    // the block's partial sums follow the lanes' state (or the tables): th[W], then
+    // (MBG_CELLS_LDS) the window table [W][2^K] and the final states [2^K]
+    u64* thl = part;
 #if MBG_CELLS_LDS
     const u32 S = 1u << D.K;
     const u32 nparts = (u32)D.W + ((u32)D.W + 1u) * S;
@@ -1172,6 +1209,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
 #endif
     }
 #else
+    (void)part;
     simulate_variant(D, tb, lane, nullptr);
 #endif
 }

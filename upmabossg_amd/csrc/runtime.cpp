@@ -137,19 +137,22 @@ struct Desc {
     void* cell_vals;
     void* cell_miss;
     uint64_t cell_mask;
+    void* lane_state;  // lanes' state in HBM (MBG_STATE_HBM)
+    uint32_t lane_stride, pad2_;
 };
 
 using mbg_desc_ptr = const void*;
 
 struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 2, scan = 4, wsum_reg = 0, rate32 = 0,
-        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0, th_lds = 0, cells_lds = 0, dep_cap = 0;
+        multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0, th_lds = 0, cells_lds = 0, dep_cap = 0,
+        state_hbm = 0;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
-                        sumw, compact, wsmad, sparse, prog, th_lds, cells_lds, dep_cap) <
+                        sumw, compact, wsmad, sparse, prog, th_lds, cells_lds, dep_cap, state_hbm) <
                std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in, o.dep_batch, o.scan,
                         o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact, o.wsmad, o.sparse, o.prog, o.th_lds,
-                        o.cells_lds, o.dep_cap);
+                        o.cells_lds, o.dep_cap, o.state_hbm);
     }
 };
 
@@ -177,6 +180,7 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_TH_LDS " << sh.th_lds << "\n";
     os << "#define MBG_CELLS_LDS " << sh.cells_lds << "\n";
     os << "#define MBG_DEP_CAP " << sh.dep_cap << "\n";
+    os << "#define MBG_STATE_HBM " << sh.state_hbm << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -308,7 +312,8 @@ struct DevBuf {
 };
 
 struct RunBuffers {
-    DevBuf<uint64_t> blob, rtab, stage, prob, th, fin, fp, fpc, summ, work, flips, cells, cell_vals, cell_miss, cell_out;
+    DevBuf<uint64_t> blob, rtab, stage, prob, th, fin, fp, fpc, summ, work, flips, cells, cell_vals, cell_miss, cell_out,
+        lane_state;
     DevBuf<float> istate;
 };
 
@@ -559,13 +564,15 @@ void launch(hipFunction_t f, uint64_t grid, int block, size_t smem, hipStream_t 
 }
 
 // Every device buffer a descriptor names must be allocated before a launch reads it.
-void check_desc(const Desc& D, bool sparse) {
+void check_desc(const Desc& D, bool sparse, bool state_hbm = false) {
     const void* need[] = {D.blob, D.istate, D.stage, D.prob, D.th, D.fin, D.fp_states, D.fp_count, D.summ,
                           D.work_counter, D.flip_total};
     for (const void* q : need)
         if (!q) throw Error(MBG_ERR_ARG, "internal error: a device buffer of the run is not allocated");
     if (sparse && (!D.cells || !D.cell_vals || !D.cell_miss))
         throw Error(MBG_ERR_ARG, "internal error: the sparse window table is not allocated");
+    if (state_hbm && (!D.lane_state || !D.lane_stride))
+        throw Error(MBG_ERR_ARG, "internal error: the lanes' state buffer is not allocated");
 }
 
 // Window count, fixed-point exponents and discrete step count of a run of n trajectories.
@@ -774,6 +781,15 @@ void read_cells(mbg_ctx* c, const Module& mod, uint64_t cap, int W, int K, hipSt
 }
 
 // The shape a run uses without measurement: occupancy model plus tuning knobs.
+// The lanes' state in HBM (MBG_STATE_HBM): occupancy is then set by registers (blocks of 256,
+// about 4 waves per SIMD), and the tables stay in LDS when 4 blocks' copies fit beside the
+// partial sums.
+void make_state_hbm(Shape& sh, uint32_t blob_bytes, int block_req) {
+    sh.state_hbm = 1;
+    sh.block = block_req ? block_req : 256;
+    sh.tables_lds = blob_bytes <= (36u << 10) ? 1 : 0;
+}
+
 Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::RateTables& rt, int block_req, int trace) {
     const bool rate32 = rates_fit_u32(rt);
     const bool compact = compact_fits(cm) && env_int("MBG_TUNE_COMPACT", 1) != 0;
@@ -782,6 +798,11 @@ Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::R
     sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
     sh.sparse = cm.K > mbg::kMaxReadouts ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
+    // lanes' state in HBM by default where one wave's state would take more than half a CU's LDS
+    // (one wave per CU in LDS)
+    const int hbm_auto = lane_lds_bytes(cm.NW, 64, sh.wsum_reg) > (80u << 10) ? 1 : 0;
+    if (env_int("MBG_TUNE_STATE_HBM", hbm_auto) == 1)
+        make_state_hbm(sh, pack_blob(cm, rt, rate32, compact).lay.bytes, block_req);
     return sh;
 }
 
@@ -812,7 +833,8 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     sh.sparse = sparse ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
     const Blob blob = pack_blob(cm, rt, sh.rate32 != 0, sh.compact != 0);
-    const size_t state_smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg) + (sh.tables_lds ? blob.lay.bytes : 0);
+    const size_t state_smem = (sh.state_hbm ? 0 : lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg)) +
+                              (sh.tables_lds ? blob.lay.bytes : 0);
     if (state_smem > (size_t)c->lds_per_block) throw Error(MBG_ERR_LIMIT, "launch shape does not fit in LDS");
     // Per-block partial sums in LDS, beside the shape's lanes and tables (they never change
     // the shape itself): the window entropies (W cells, summed by every trajectory in every
@@ -892,6 +914,23 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     D.stage = d_stage.p;
     D.fp_states = d_fp.p;
     D.work_counter = d_work.p;
+    // lanes' state in HBM: one column per thread of the largest grid a batch launches.  The
+    // lanes' accesses are scattered 8-byte words, so the state is kept within about half of the
+    // 256 MB Infinity Cache (fewer resident lanes, but cache hits instead of HBM lines): C's
+    // 79-word lanes ran 307 ms at 65,536 lanes (124 MB) and 459 ms at 262,144 (497 MB).
+    uint64_t grid_max = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)blocks_per_cu;
+    if (sh.state_hbm && !p->grid) {
+        const uint64_t lane_bytes = (uint64_t)(sh.wsum_reg == 1 ? 2 : 3) * (uint64_t)cm.NW * 8;
+        const uint64_t fit = (uint64_t)env_int("MBG_TUNE_STATE_MB", 128) * (1ull << 20) / (lane_bytes * (uint64_t)sh.block);
+        grid_max = std::max<uint64_t>((uint64_t)c->cus, std::min<uint64_t>(grid_max, fit));
+    }
+    if (sh.state_hbm) {
+        const uint64_t stride = grid_max * (uint64_t)sh.block;
+        if (stride > 0xffffffffull / (3ull * (uint64_t)cm.NW)) throw Error(MBG_ERR_LIMIT, "grid too large for the lanes' state");
+        B.lane_state.alloc((size_t)stride * (sh.wsum_reg == 1 ? 2 : 3) * cm.NW);
+        D.lane_state = B.lane_state.p;
+        D.lane_stride = (uint32_t)stride;
+    }
     D.fp_cap = fp_cap;
     D.summ_cap = summ_cap;
     D.nsteps = nsteps;
@@ -936,7 +975,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
         D.cell_vals = sparse ? B.cell_vals.p : nullptr;
         D.cell_miss = sparse ? B.cell_miss.p : nullptr;
         D.cell_mask = sparse ? cell_cap - 1 : 0;
-        check_desc(D, sparse);
+        check_desc(D, sparse, sh.state_hbm != 0);
         sim_ms = 0.f;
         for (uint64_t done = 0; done < count; done += D.batch) {
             D.batch = std::min(batch_max, count - done);
@@ -948,7 +987,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
             if (cm.NW > 128) launch(mod.init_state, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_state");
             launch(mod.init_rates, rate_blocks, 256, 0, s, D, "launch mbg_init_rates");
             launch(mod.init_readouts, (D.batch + 255) / 256, 256, 0, s, D, "launch mbg_init_readouts");
-            uint64_t grid = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)blocks_per_cu;
+            uint64_t grid = grid_max;
             grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (D.batch + sh.block - 1) / sh.block));
             hip_check(hipEventRecord(e0, s), "hipEventRecord");
             launch(mod.sim, grid, sh.block, smem, s, D, "launch mbg_simulate");
@@ -1081,6 +1120,10 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
             add(base.compact, base.wsum_reg, base.tables_lds, waves);
         }
         for (const Shape& sh : cands) consider(sh);
+        // the lanes' state in HBM, tables as the default places them
+        Shape h = base;
+        make_state_hbm(h, pack_blob(cm, rt, r32, base.compact != 0).lay.bytes, 0);
+        consider(h);
     }
     // step knobs on the best layout
     {
@@ -1124,6 +1167,7 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
         out->best_ms = best_ms;
         out->default_ms = base_ms;
         out->dep_cap = best.dep_cap;
+        out->state_hbm = best.state_hbm;
     }
 }
 
