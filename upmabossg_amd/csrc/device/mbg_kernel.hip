@@ -379,6 +379,46 @@ __device__ __forceinline__ u64 rate_of(const mbg_desc& D, const char* tb, const 
     return rate_from_key(D, tb, n, i, key);
 }
 
+// Entropy term rint(r log2(r) 2^sigma) of readout ro (index) at rate r produced with own state
+// bit xi: a table entry for fast nodes, computed for generic ones.
+__device__ __forceinline__ u64 entropy_term(const mbg_desc& D, const char* tb, const mbg_nd& n, u32 ro, u32 xi,
+                                            u64 r) {
+    if (!r) return 0ull;
+#if MBG_GENERIC
+    if (n.flags & F_GEN) {
+        const double rd = (double)r;
+        return (u64)rint(ldexp(rd * mbg_log2(rd), D.sigma));
+    }
+#else
+    (void)D;
+    (void)n;
+#endif
+    return TTERM(2 * ro + xi);
+}
+
+// Readout part of a state (word(w) gives state word w): projected state, readout rate sum,
+// entropy-term sum and active-readout count, over readouts j0, j0 + step, ...
+template <class Word>
+__device__ __forceinline__ void readout_sums(const mbg_desc& D, const char* tb, int j0, int step, Word word, u64& s,
+                                             u64& R, u64& S, u64& act) {
+    s = R = S = act = 0;
+    for (int j = j0; j < D.K; j += step) {
+        const u32 n = TRO(j);
+        const mbg_nd nd = TNODE(n);
+        u32 key = (u32)(word(n >> 6) >> (n & 63)) & 1u;
+        for (u32 m = 0; m < IN_CNT(nd); m++) {
+            const u32 q = TIN(nd.in_off + m);
+            key |= ((u32)(word(q >> 6) >> (q & 63)) & 1u) << (m + 1);
+        }
+        const u64 r = rate_of(D, tb, nd, n, key, word);
+        const u32 xi = key & 1u;
+        s |= (u64)xi << j;
+        R += r;
+        S += entropy_term(D, tb, nd, (u32)j, xi, r);
+        act += r != 0 ? 1ull : 0ull;
+    }
+}
+
 // Initial states, active sets and word rate sums.  A block takes 64 trajectories: it draws
 // their state words into LDS (lane-interleaved) and the staging area, then each wave sweeps
 // whole words for the 64 trajectories, one per lane, so the table reads of a word's 64
@@ -504,6 +544,25 @@ __device__ __forceinline__ void init_rates_block(const mbg_desc& D, u64 t0) {
             D.stage[t * STAGE_ROW + 2 * MBG_NW + w] = sum;
         }
     }
+    // the readout part of each staged state: wave q sums readouts q, q + 4, ... of the
+    // trajectory in its lane, then the first wave adds the four partial sums
+    __shared__ u64 rp[4][4][INIT_TRAJ];
+    {
+        const u32 q = tid >> 6;
+        u64 s, R, S, act;
+        readout_sums(D, tb, (int)q, 4, [&](u32 w) { return xs[w * INIT_TRAJ + lane]; }, s, R, S, act);
+        rp[q][0][lane] = s;
+        rp[q][1][lane] = R;
+        rp[q][2][lane] = S;
+        rp[q][3][lane] = act;
+    }
+    __syncthreads();
+    if (tid < INIT_TRAJ && t0 + tid < D.batch) {
+        u64* row = D.stage + (t0 + tid) * STAGE_ROW + 3 * MBG_NW;
+        row[0] = rp[0][0][tid] | rp[1][0][tid] | rp[2][0][tid] | rp[3][0][tid];
+#pragma unroll
+        for (int f = 1; f < 4; f++) row[f] = rp[0][f][tid] + rp[1][f][tid] + rp[2][f][tid] + rp[3][f][tid];
+    }
 }
 
 extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
@@ -549,52 +608,14 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
 }
 #endif
 
-// Entropy term rint(r log2(r) 2^sigma) of readout ro (index) at rate r produced with own state
-// bit xi: a table entry for fast nodes, computed for generic ones.
-__device__ __forceinline__ u64 entropy_term(const mbg_desc& D, const char* tb, const mbg_nd& n, u32 ro, u32 xi,
-                                            u64 r) {
-    if (!r) return 0ull;
-#if MBG_GENERIC
-    if (n.flags & F_GEN) {
-        const double rd = (double)r;
-        return (u64)rint(ldexp(rd * mbg_log2(rd), D.sigma));
-    }
-#else
-    (void)D;
-    (void)n;
-#endif
-    return TTERM(2 * ro + xi);
-}
-
-// Pass 3, one thread per staged trajectory: the readout part of its initial state (projected
-// state, readout rate sum, entropy-term sum, active-readout count) at the end of its stage row,
-// so that a lane starting the trajectory reads four words instead of looping over the readouts
-// while the rest of its wave waits.
+// Models above 128 words, pass 3, one thread per staged trajectory: the readout part of its
+// initial state at the end of its stage row (smaller models get it from mbg_init_rates), so that
+// a lane starting the trajectory reads four words instead of looping over the readouts while the
+// rest of its wave waits.
 __device__ __forceinline__ void init_readouts_one(const mbg_desc& D, u64 t) {
     u64* row = D.stage + t * STAGE_ROW;
-    const char* tb = D.blob;
-    u64 s = 0, R = 0, S = 0, act = 0;
-    for (int j = 0; j < D.K; j++) {
-        const u32 n = TRO(j);
-        const u32 xi = (u32)(row[n >> 6] >> (n & 63)) & 1u;
-        s |= (u64)xi << j;
-        const mbg_nd nd = TNODE(n);
-        u64 r;
-#if MBG_GENERIC
-        if (nd.flags & F_GEN) {
-            u32 key = xi;
-            for (u32 m = 0; m < IN_CNT(nd); m++) {
-                const u32 q = TIN(nd.in_off + m);
-                key |= ((u32)(row[q >> 6] >> (q & 63)) & 1u) << (m + 1);
-            }
-            r = rate_from_key(D, tb, nd, n, key);
-        } else
-#endif
-            r = ((row[MBG_NW + (n >> 6)] >> (n & 63)) & 1ull) ? TRATE(2 * n + xi) : 0ull;
-        R += r;
-        S += entropy_term(D, tb, nd, (u32)j, xi, r);
-        act += r != 0 ? 1ull : 0ull;
-    }
+    u64 s, R, S, act;
+    readout_sums(D, D.blob, 0, 1, [&](u32 w) { return row[w]; }, s, R, S, act);
     row[3 * MBG_NW + 0] = s;
     row[3 * MBG_NW + 1] = R;
     row[3 * MBG_NW + 2] = S;
@@ -606,15 +627,6 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_readouts(mbg_desc D) 
     if (t < D.batch) init_readouts_one(D, t);
 }
 
-#if MBG_MULTI
-// The same for every variant of a sweep class: block (x, v) takes trajectories 256x ..
-// 256x + 255 of variant v.
-extern "C" __global__ __launch_bounds__(256) void mbg_init_readouts_multi(const mbg_desc* descs) {
-    const mbg_desc& D = descs[blockIdx.y];
-    const u64 t = (u64)blockIdx.x * 256u + threadIdx.x;
-    if (t < D.batch) init_readouts_one(D, t);
-}
-#endif
 
 // ------------------------------------------------------------------ main kernel
 

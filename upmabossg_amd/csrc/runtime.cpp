@@ -275,8 +275,8 @@ Blob pack_blob(const mbg::CompiledModel& cm, const mbg::RateTables& rt, bool rat
 struct Module {
     hipModule_t mod = nullptr;
     hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, init_readouts = nullptr,
-                  init_readouts_multi = nullptr, marginals = nullptr, marginals_batch = nullptr,
-                  init_rates_multi = nullptr, fp_dedup = nullptr, fp_compact = nullptr,
+                  marginals = nullptr, marginals_batch = nullptr, init_rates_multi = nullptr, fp_dedup = nullptr,
+                  fp_compact = nullptr,
                   fp_gather = nullptr, cells_count = nullptr, cells_compact = nullptr;
 };
 
@@ -340,8 +340,7 @@ struct MargJob {
 struct SweepClass {
     Shape sh{};
     hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, init_readouts = nullptr,
-                  init_readouts_multi = nullptr, marginals = nullptr, marginals_batch = nullptr,
-                  init_rates_multi = nullptr;
+                  marginals = nullptr, marginals_batch = nullptr, init_rates_multi = nullptr;
     std::vector<int> members;
     DevBuf<Desc> descs;
     size_t smem = 0;
@@ -437,9 +436,6 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
     hip_check(hipModuleGetFunction(&m.init_state, m.mod, "mbg_init_state"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_rates, m.mod, "mbg_init_rates"), "hipModuleGetFunction");
     hip_check(hipModuleGetFunction(&m.init_readouts, m.mod, "mbg_init_readouts"), "hipModuleGetFunction");
-    if (sh.multi)
-        hip_check(hipModuleGetFunction(&m.init_readouts_multi, m.mod, "mbg_init_readouts_multi"),
-                  "hipModuleGetFunction");
     return c->modules.emplace(sh, m).first->second;
 }
 
@@ -986,7 +982,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
             const uint64_t rate_blocks = cm.NW <= 128 ? (D.batch + 63) / 64 : (init_threads + 255) / 256;
             if (cm.NW > 128) launch(mod.init_state, (init_threads + 255) / 256, 256, 0, s, D, "launch mbg_init_state");
             launch(mod.init_rates, rate_blocks, 256, 0, s, D, "launch mbg_init_rates");
-            launch(mod.init_readouts, (D.batch + 255) / 256, 256, 0, s, D, "launch mbg_init_readouts");
+            if (cm.NW > 128) launch(mod.init_readouts, (D.batch + 255) / 256, 256, 0, s, D, "launch mbg_init_readouts");
             uint64_t grid = grid_max;
             grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (D.batch + sh.block - 1) / sh.block));
             hip_check(hipEventRecord(e0, s), "hipEventRecord");
@@ -1286,7 +1282,6 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         C.init_state = mod.init_state;
         C.init_rates = mod.init_rates;
         C.init_readouts = mod.init_readouts;
-        C.init_readouts_multi = mod.init_readouts_multi;
         C.marginals = mod.marginals;
         C.marginals_batch = mod.marginals_batch;
         C.init_rates_multi = mod.init_rates_multi;
@@ -1419,11 +1414,6 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             hip_check(hipModuleLaunchKernel(C.init_rates_multi, (unsigned)bx, (unsigned)C.members.size(), 1, 256, 1, 1, 0,
                                             st, iargs, nullptr),
                       "launch mbg_init_rates_multi");
-            uint64_t rx = 1;
-            for (int v : C.members) rx = std::max<uint64_t>(rx, (sw->vars[v]->D.batch + 255) / 256);
-            hip_check(hipModuleLaunchKernel(C.init_readouts_multi, (unsigned)rx, (unsigned)C.members.size(), 1, 256, 1,
-                                            1, 0, st, iargs, nullptr),
-                      "launch mbg_init_readouts_multi");
         } else {
             for (int v : C.members) {
                 Desc& D = sw->vars[v]->D;
