@@ -37,16 +37,19 @@
 //   wsum[w] the exact fixed-point rate sum of each 64-node word,
 // x and act in LDS, lane-interleaved (element w of lane l at w*BLOCK + l: no two lanes of
 // a wave share a bank whatever word each touches), wsum in VGPRs (MBG_WSUM_REG, read and
-// written through unrolled selects) or in LDS the same way.  A transition re-evaluates
-// only the flipped node and its dependents from per-node truth tables and applies exact
-// integer deltas, so the two-level search (word sums, then the active bits of one word)
-// selects exactly the node the reference's linear scan of prefix sums would.
+// written through unrolled selects) or in LDS the same way; for wide models all of it in an
+// HBM buffer interleaved the same way over the grid (MBG_STATE_HBM).  A transition
+// re-evaluates only the flipped node and its dependents from per-node truth tables and
+// applies exact integer deltas, so the two-level search (word sums, then the active bits of
+// one word) selects exactly the node the reference's linear scan of prefix sums would.  The
+// dependents are re-evaluated at the start of the next loop iteration, at most MBG_DEP_CAP of
+// them per iteration, so one lane with many dependents does not hold its whole wave.
 //
 // Initial states are built by a throughput-bound pre-pass (mbg_init_rates: the initial
-// draws and every node's rate for 64 trajectories per block, staged through LDS; above 128
-// words mbg_init_state draws the states first) into an HBM staging area, so a lane that
-// starts a trajectory only copies 3*NW words instead of evaluating all N nodes while its
-// wave waits.
+// draws and every node's rate for 64 trajectories per block, staged through LDS, then the
+// readout sums; above 128 words mbg_init_state draws the states first and mbg_init_readouts
+// sums the readouts after) into an HBM staging area, so a lane that starts a trajectory only
+// copies 3*NW + 4 words instead of evaluating all N nodes while its wave waits.
 //
 // Statistics (src/jit_kernels/window_average_small.cu:50-136, final_states.cu:9-21,
 // src/statistics/fixed_states.cu:84-125) are accumulated per lane in registers for the
