@@ -179,6 +179,16 @@ def test_lane_state_in_hbm_bit_exact(gpu_ctx, monkeypatch, case):
         _compare(gpu_ctx, net.bnd(), net.cfg(), count=512, n=262144)
 
 
+def test_more_than_128_words_bit_exact(gpu_ctx):
+    """8,300 nodes (130 words): the initial states are drawn per (trajectory, word), the readout
+    part of each staged state by its own pass, and the lanes' state (3,120 bytes per lane, more
+    than LDS holds for a wave) lives in HBM.  A short run keeps the O(N)-per-step oracle quick."""
+    net = synth.random_network(8300, 1, 3, 31, rates="loguniform", istate="uniform", readouts=10, special_frac=0.2,
+                               max_time=0.5, time_tick=0.125, sample_count=64)
+    g, r = _compare(gpu_ctx, net.bnd(), net.cfg(), count=64)
+    assert gpu_ctx.info().state_words == 130 and g.transitions > 10000
+
+
 def test_block_size_and_grid_independent(gpu_ctx):
     net = synth.config_small()
     m = Model(net.bnd(), net.cfg())

@@ -497,7 +497,7 @@ int default_dep_cap(const mbg::CompiledModel& cm, int dep_batch) {
 // VGPRs).  Without tuning knobs: stage the tables in LDS when they fit beside the lanes'
 // state and keep the most waves resident per CU (160 KiB of LDS per CU on gfx950).
 Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob_bytes, int block_req, int trace,
-                   int rate32) {
+                   int rate32, bool lanes_in_lds = true) {
     const size_t cu_lds = 160 * 1024;
     const int max_in = std::max(1, cm.max_inputs);
     // word sums in VGPRs pay off while the unrolled select over the words stays short
@@ -527,7 +527,7 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
     best.dep_cap = default_dep_cap(cm, best.dep_batch);
     best.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
     best.sumw = default_sumw(cm.NW, best.wsum_reg);
-    if (best_score < 0) {
+    if (best_score < 0 && lanes_in_lds) {
         if (block_req) throw Error(MBG_ERR_LIMIT, "the requested block does not fit in LDS for this model");
         throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
     }
@@ -789,16 +789,17 @@ void make_state_hbm(Shape& sh, uint32_t blob_bytes, int block_req) {
 Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::RateTables& rt, int block_req, int trace) {
     const bool rate32 = rates_fit_u32(rt);
     const bool compact = compact_fits(cm) && env_int("MBG_TUNE_COMPACT", 1) != 0;
-    Shape sh = choose_shape(c, cm, pack_blob(cm, rt, rate32, compact).lay.bytes, block_req, trace, rate32 ? 1 : 0);
+    const uint32_t blob_bytes = pack_blob(cm, rt, rate32, compact).lay.bytes;
+    // lanes' state in HBM by default where one wave's state would take more than half a CU's LDS
+    // (one wave per CU in LDS; above about 106 words it would not fit at all)
+    const int hbm_auto = lane_lds_bytes(cm.NW, 64, default_wsum_reg(cm.NW)) > (80u << 10) ? 1 : 0;
+    const bool hbm = env_int("MBG_TUNE_STATE_HBM", hbm_auto) == 1;
+    Shape sh = choose_shape(c, cm, blob_bytes, block_req, trace, rate32 ? 1 : 0, !hbm);
     sh.compact = compact ? 1 : 0;
     sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
     sh.sparse = cm.K > mbg::kMaxReadouts ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
-    // lanes' state in HBM by default where one wave's state would take more than half a CU's LDS
-    // (one wave per CU in LDS)
-    const int hbm_auto = lane_lds_bytes(cm.NW, 64, sh.wsum_reg) > (80u << 10) ? 1 : 0;
-    if (env_int("MBG_TUNE_STATE_HBM", hbm_auto) == 1)
-        make_state_hbm(sh, pack_blob(cm, rt, rate32, compact).lay.bytes, block_req);
+    if (hbm) make_state_hbm(sh, blob_bytes, block_req);
     return sh;
 }
 
