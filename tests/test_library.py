@@ -175,6 +175,9 @@ def test_generated_source_is_specialised():
     assert "#define MBG_NW 16" in src
     assert "#define MBG_DISCRETE 0" in src
     assert "mbg_simulate" in src and "mbg_philox" in src
+    # launch-shape knobs are compile-time constants of the generated source
+    assert "#define MBG_DEP_CAP " in src and "#define MBG_STATE_HBM " in src
+    assert "mbg_init_readouts" in src  # the readout pass of models above 128 words
     b, c = model_text("cellcycle")
     src = Model(b, c).generate_source(trace=True)
     assert "#define MBG_NW 1" in src and "#define MBG_DISCRETE 1" in src and "#define MBG_TRACE 1" in src
