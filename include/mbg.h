@@ -123,7 +123,7 @@ typedef struct mbg_tune_info {
     uint64_t sample;
     double best_ms, default_ms;   /* kernel time of the chosen and of the untuned shape */
     int dep_cap;                  /* dependents re-evaluated per loop iteration (0: all) */
-    int state_hbm;                /* 1: the lanes' state in HBM instead of LDS */
+    int state_hbm;                /* 1: the lanes' state in HBM instead of LDS, 2: only the active sets */
 } mbg_tune_info;
 mbg_status mbg_autotune(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, mbg_tune_info* out);
 
@@ -137,6 +137,7 @@ typedef struct mbg_run_info {
     double kernel_ms;          /* mbg_simulate time (HIP events), all batches */
     float time_tick, max_time;
     int block, tables_in_lds, lds_bytes, blocks_per_cu;  /* launch shape chosen for the run */
+    int state_hbm;             /* lanes' state in HBM: 0 none (LDS), 1 all of it, 2 the active sets */
 } mbg_run_info;
 mbg_status mbg_get_run_info(const mbg_ctx* c, mbg_run_info* out);
 

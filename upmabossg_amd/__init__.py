@@ -69,6 +69,7 @@ class _RunInfo(C.Structure):
         ("fixed_point_trajectories", C.c_uint64), ("kernel_ms", C.c_double),
         ("time_tick", C.c_float), ("max_time", C.c_float),
         ("block", C.c_int), ("tables_in_lds", C.c_int), ("lds_bytes", C.c_int), ("blocks_per_cu", C.c_int),
+        ("state_hbm", C.c_int),
     ]
 
 

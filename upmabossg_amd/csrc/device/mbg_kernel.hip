@@ -25,7 +25,9 @@
 //   MBG_CELLS_LDS   1: also of the whole window table and final states (small tables)
 //   MBG_STATE_HBM   1: the lanes' x, act and word sums in an HBM buffer (lane-interleaved over
 //                   the whole grid) instead of LDS, for models whose per-lane state would leave
-//                   a CU with one or two waves (single runs only)
+//                   a CU with one or two waves (single runs only); 2: only act in that buffer,
+//                   x in LDS and the word sums in VGPRs (half the LDS per lane, so twice the
+//                   resident waves; act is read once per step and once per rate change)
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -118,6 +120,9 @@ typedef unsigned int sidx;
 #endif
 #if MBG_STATE_HBM && MBG_MULTI
 #error "the lane state in HBM is a layout of single runs"
+#endif
+#if MBG_STATE_HBM == 2 && MBG_WSUM_REG != 1
+#error "the active sets alone in HBM need the word sums in VGPRs"
 #endif
 
 // A node record decoded into registers (either table layout).
@@ -270,13 +275,18 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
 
 // Lane state: element w of a lane's array at w * LSTRIDE + tid (no two lanes of a wave share
 // an LDS bank whatever word each touches; in HBM, consecutive lanes share cache lines).
-#if MBG_STATE_HBM
+#if MBG_STATE_HBM == 1
 #define LSTRIDE D.lane_stride
 #else
 #define LSTRIDE MBG_BLOCK
 #endif
 #define LANE_X(w) lane[(0 * MBG_NW + (w)) * LSTRIDE + tid]
+#if MBG_STATE_HBM == 2
+// act alone in HBM, word w of grid thread atid at w * lane_stride + atid
+#define LANE_A(w) D.lane_state[(u64)(w) * D.lane_stride + atid]
+#else
 #define LANE_A(w) lane[(1 * MBG_NW + (w)) * LSTRIDE + tid]
+#endif
 #if MBG_WSUM_REG
 // rate sums of groups of MBG_SUMW words in registers: every access is an unrolled select
 // over the groups (at most 16)
@@ -733,7 +743,10 @@ __device__ __forceinline__ void flush_th(const mbg_desc& D, u64* thl, int w, u64
 
 __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* tb, u64* lane, u64* thl) {
     // index of this lane's state: its thread in the block (LDS) or in the grid (HBM)
-    const u32 tid = MBG_STATE_HBM ? blockIdx.x * MBG_BLOCK + threadIdx.x : threadIdx.x;
+    const u32 tid = MBG_STATE_HBM == 1 ? blockIdx.x * MBG_BLOCK + threadIdx.x : threadIdx.x;
+#if MBG_STATE_HBM == 2
+    const u32 atid = blockIdx.x * MBG_BLOCK + threadIdx.x;  // this lane's column of act in HBM
+#endif
     const u32 lid = threadIdx.x & 63u;
     Lane L{D, tb, lane, tid};
     const u32 key0 = (u32)D.seed, key1 = (u32)(D.seed >> 32);
@@ -845,6 +858,9 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #endif
             for (u32 q0 = 0; q0 < nq; q0 += DEP_BATCH) {
                 u32 kk[DEP_BATCH], xo[DEP_BATCH], xn[DEP_BATCH];
+#if MBG_STATE_HBM == 2
+                u64 aw[DEP_BATCH];
+#endif
                 u64 rn[DEP_BATCH], rp[DEP_BATCH];
                 mbg_nd nk[DEP_BATCH];
 #pragma unroll
@@ -853,6 +869,11 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     const u32 dp = TDEP(pend_off + (ok ? q0 + u : 0u));
                     kk[u] = dp & 0xffffffu;
                     const u32 pos = dp >> 24;
+#if MBG_STATE_HBM == 2
+                    // the act word is read from HBM before the key gather, whose LDS reads
+                    // and logic then cover its latency
+                    aw[u] = LANE_A(kk[u] >> 6);
+#endif
                     nk[u] = TNODE(kk[u]);
                     const u32 key = L.key_of(nk[u], kk[u]);
                     xn[u] = key & 1u;
@@ -866,7 +887,11 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     {
                         // both table reads unconditional (valid entries), then masked
                         const u32 logic = L.logic_of(nk[u], key);
+#if MBG_STATE_HBM == 2
+                        const u32 a_old = (u32)(aw[u] >> (kk[u] & 63)) & 1u;
+#else
                         const u32 a_old = (u32)(LANE_A(kk[u] >> 6) >> (kk[u] & 63)) & 1u;
+#endif
                         const u64 r_new = TRATE(2 * kk[u] + xn[u]), r_old = TRATE(2 * kk[u] + xo[u]);
                         rn[u] = xn[u] != logic ? r_new : 0ull;
                         rp[u] = a_old ? r_old : 0ull;
@@ -880,8 +905,17 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                         const u64 bit = 1ull << (kk[u] & 63);
                         WS_ADD(wk, rn[u] - rp[u]);
                         total += rn[u] - rp[u];
+#if MBG_STATE_HBM == 2
+                        // act words of two dependents in one batch may coincide: the batch's
+                        // earlier writes are in memory, so read the word again only then
+                        u64 a = aw[u];
+                        _Pragma("unroll") for (int v = 0; v < u; v++) if ((kk[v] >> 6) == wk) a = LANE_A(wk);
+                        const u64 an = rn[u] ? (a | bit) : (a & ~bit);
+                        if (an != a) LANE_A(wk) = an;
+#else
                         const u64 a = LANE_A(wk);
                         LANE_A(wk) = rn[u] ? (a | bit) : (a & ~bit);
+#endif
                         if (nk[u].flags & F_RO) {
                             const u32 ro = (nk[u].flags & F_RO) - 1u;
                             ro_R += rn[u] - rp[u];
@@ -1090,7 +1124,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         u64 lim = down ? sgs - rem0 - 1ull : rem0;
         // guard for an inconsistent group (never taken when the sums are exact)
         u32 j = w0 * 64u;
-        u64 r_sel = 0, xw = LANE_X(j >> 6);
+        u64 r_sel = 0, xw = LANE_X(j >> 6), a_hit = 0;
         bool hit = false;
         for (int q0 = 0; q0 < SEARCH_W && !hit; q0++) {
             const int q = down ? SEARCH_W - 1 - q0 : q0;
@@ -1137,6 +1171,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 }
                 if (hit) {
                     xw = xq;
+                    a_hit = am;
                     break;
                 }
                 m = mm;
@@ -1153,7 +1188,8 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             const u32 wj = j >> 6;
             WS_ADD(wj, 0ull - r_sel);
             total -= r_sel;
-            LANE_A(wj) = LANE_A(wj) & ~(1ull << (j & 63));
+            // the hit's act word was read by the search (HBM act: no second read)
+            LANE_A(wj) = (MBG_STATE_HBM == 2 && hit ? a_hit : LANE_A(wj)) & ~(1ull << (j & 63));
             if (nj.flags & F_RO) {
                 ro_R -= r_sel;
                 ro_S -= L.term(nj, (nj.flags & F_RO) - 1u, xj_old, r_sel);
@@ -1190,12 +1226,12 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
     const char* tb = D.blob;
     u64* after = smem;
 #endif
-#if MBG_STATE_HBM
+#if MBG_STATE_HBM == 1
     u64* lane = D.lane_state;
     u64* part = after;
 #else
     u64* lane = after;
-    u64* part = lane + (MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
+    u64* part = lane + (MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
 #endif
 #if MBG_TH_LDS
     // the block's partial sums follow the lanes' state (or the tables): th[W], then

@@ -443,8 +443,12 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
 // state, readout rate sum, entropy-term sum, active-readout count), as STAGE_ROW in the kernel.
 uint64_t stage_row(int nw) { return 3ull * (uint64_t)nw + 4ull; }
 
-// LDS of the lanes' state: x and act, plus the word rate sums unless they are in VGPRs only
-size_t lane_lds_bytes(int nw, int block, int wsum_reg) { return (size_t)(wsum_reg == 1 ? 2 : 3) * nw * block * 8; }
+// LDS of the lanes' state: x and act, plus the word rate sums unless they are in VGPRs only;
+// x alone with act in HBM (state_hbm 2), nothing with the whole state in HBM (state_hbm 1)
+size_t lane_lds_bytes(int nw, int block, int wsum_reg, int state_hbm = 0) {
+    if (state_hbm == 1) return 0;
+    return (size_t)(state_hbm == 2 ? 1 : wsum_reg == 1 ? 2 : 3) * nw * block * 8;
+}
 
 int env_int(const char* name, int dflt) {
     const char* e = std::getenv(name);
@@ -537,11 +541,11 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
 // The block with the most resident waves for a layout (word-sum placement, tables in LDS or
 // not), optionally only among blocks with fewer than below_waves waves; 0 if nothing fits.
 int layout_block(const mbg_ctx* c, int nw, int wsum_reg, uint32_t blob_bytes, int tables, int* waves_out,
-                 int below_waves = 1 << 30) {
+                 int below_waves = 1 << 30, int state_hbm = 0) {
     const size_t cu_lds = 160 * 1024;
     int best = 0, best_waves = 0;
     for (int block = 1024; block >= 64; block -= 64) {
-        const size_t lds = lane_lds_bytes(nw, block, wsum_reg) + (tables ? blob_bytes : 0);
+        const size_t lds = lane_lds_bytes(nw, block, wsum_reg, state_hbm) + (tables ? blob_bytes : 0);
         if (lds > (size_t)c->lds_per_block) continue;
         const int waves = std::min(16, (int)(cu_lds / lds) * (block / 64));
         if (waves >= below_waves) continue;
@@ -793,13 +797,52 @@ Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::R
     // lanes' state in HBM by default where one wave's state would take more than half a CU's LDS
     // (one wave per CU in LDS; above about 106 words it would not fit at all)
     const int hbm_auto = lane_lds_bytes(cm.NW, 64, default_wsum_reg(cm.NW)) > (80u << 10) ? 1 : 0;
-    const bool hbm = env_int("MBG_TUNE_STATE_HBM", hbm_auto) == 1;
-    Shape sh = choose_shape(c, cm, blob_bytes, block_req, trace, rate32 ? 1 : 0, !hbm);
+    int hbm_mode = env_int("MBG_TUNE_STATE_HBM", hbm_auto);
+    if (hbm_mode == 0 && !std::getenv("MBG_TUNE_STATE_HBM") && !block_req && default_wsum_reg(cm.NW) == 1) {
+        // act alone in HBM by default where it keeps more waves resident (up to 12 per CU, which
+        // leave the step loop its VGPRs): the 1,000-node network of 16 words runs 12 waves
+        // instead of 8, 64 ms instead of 74 per million trajectories
+        const Shape lds = choose_shape(c, cm, blob_bytes, 0, trace, rate32 ? 1 : 0, true);
+        const size_t lds_bytes = lane_lds_bytes(cm.NW, lds.block, lds.wsum_reg) + (lds.tables_lds ? blob_bytes : 0);
+        const int waves0 = std::min(16, (int)(160 * 1024 / lds_bytes) * (lds.block / 64));
+        int waves2 = 0;
+        layout_block(c, cm.NW, 1, blob_bytes, lds.tables_lds, &waves2, 13, 2);
+        if (waves2 > waves0) hbm_mode = 2;
+    }
+    const bool hbm = hbm_mode == 1;
+    // (act alone in HBM sets its own block below: the LDS-layout pass only picks the rest)
+    const bool act_hbm = hbm_mode == 2 && default_wsum_reg(cm.NW) == 1;
+    Shape sh = choose_shape(c, cm, blob_bytes, act_hbm ? 64 : block_req, trace, rate32 ? 1 : 0, !hbm);
     sh.compact = compact ? 1 : 0;
     sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
     sh.sparse = cm.K > mbg::kMaxReadouts ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
     if (hbm) make_state_hbm(sh, blob_bytes, block_req);
+    if (act_hbm && sh.wsum_reg == 1) {
+        // act alone in HBM: x's LDS sets the block (a requested one if it fits), tables in LDS
+        // where they fit beside it
+        const int breq = block_req ? block_req : env_int("MBG_TUNE_BLOCK", 0);
+        for (int tables = 1; tables >= 0; tables--) {
+            int waves = 0;
+            int block = breq ? breq : layout_block(c, cm.NW, 1, blob_bytes, tables, &waves, 13, 2);
+            // (small lanes reach more than 12 LDS waves with any block: then the fullest one)
+            if (!block) block = layout_block(c, cm.NW, 1, blob_bytes, tables, &waves, 1 << 30, 2);
+            if (block && lane_lds_bytes(cm.NW, block, 1, 2) + (tables ? blob_bytes : 0) <= (size_t)c->lds_per_block) {
+                sh.state_hbm = 2;
+                sh.block = block;
+                sh.tables_lds = tables;
+                break;
+            }
+        }
+        if (sh.state_hbm != 2 && block_req) {
+            // as requested, but in the layout that fits it
+            sh = choose_shape(c, cm, blob_bytes, block_req, trace, rate32 ? 1 : 0, true);
+            sh.compact = compact ? 1 : 0;
+            sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
+            sh.sparse = cm.K > mbg::kMaxReadouts ? 1 : 0;
+            sh.prog = cm.any_prog ? 1 : 0;
+        }
+    }
     return sh;
 }
 
@@ -830,7 +873,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     sh.sparse = sparse ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
     const Blob blob = pack_blob(cm, rt, sh.rate32 != 0, sh.compact != 0);
-    const size_t state_smem = (sh.state_hbm ? 0 : lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg)) +
+    const size_t state_smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg, sh.state_hbm) +
                               (sh.tables_lds ? blob.lay.bytes : 0);
     if (state_smem > (size_t)c->lds_per_block) throw Error(MBG_ERR_LIMIT, "launch shape does not fit in LDS");
     // Per-block partial sums in LDS, beside the shape's lanes and tables (they never change
@@ -917,14 +960,14 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     // 79-word lanes ran 307 ms at 65,536 lanes (124 MB) and 459 ms at 262,144 (497 MB).
     uint64_t grid_max = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)blocks_per_cu;
     if (sh.state_hbm && !p->grid) {
-        const uint64_t lane_bytes = (uint64_t)(sh.wsum_reg == 1 ? 2 : 3) * (uint64_t)cm.NW * 8;
+        const uint64_t lane_bytes = (uint64_t)(sh.state_hbm == 2 ? 1 : sh.wsum_reg == 1 ? 2 : 3) * (uint64_t)cm.NW * 8;
         const uint64_t fit = (uint64_t)env_int("MBG_TUNE_STATE_MB", 128) * (1ull << 20) / (lane_bytes * (uint64_t)sh.block);
         grid_max = std::max<uint64_t>((uint64_t)c->cus, std::min<uint64_t>(grid_max, fit));
     }
     if (sh.state_hbm) {
         const uint64_t stride = grid_max * (uint64_t)sh.block;
         if (stride > 0xffffffffull / (3ull * (uint64_t)cm.NW)) throw Error(MBG_ERR_LIMIT, "grid too large for the lanes' state");
-        B.lane_state.alloc((size_t)stride * (sh.wsum_reg == 1 ? 2 : 3) * cm.NW);
+        B.lane_state.alloc((size_t)stride * (sh.state_hbm == 2 ? 1 : sh.wsum_reg == 1 ? 2 : 3) * cm.NW);
         D.lane_state = B.lane_state.p;
         D.lane_stride = (uint32_t)stride;
     }
@@ -1040,6 +1083,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     I.tables_in_lds = sh.tables_lds;
     I.lds_bytes = (int)smem;
     I.blocks_per_cu = blocks_per_cu;
+    I.state_hbm = sh.state_hbm;
     c->marg_fn = mod.marginals;
     c->have = true;
 }
@@ -1097,6 +1141,7 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
             const int block = layout_block(c, cm.NW, wreg, blob, tables, &waves, below);
             if (!block) return;
             Shape sh = base;
+            sh.state_hbm = 0;  // these layouts keep the lanes' whole state in LDS
             sh.compact = compact;
             sh.wsum_reg = wreg;
             sh.sumw = default_sumw(cm.NW, wreg);
@@ -1105,6 +1150,7 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
             sh.block = block;
             cands.push_back(sh);
         };
+        if (base.state_hbm == 2) add(base.compact, base.wsum_reg, base.tables_lds, 1 << 30);
         add(base.compact, base.wsum_reg, !base.tables_lds, 1 << 30);
         add(!base.compact, base.wsum_reg, base.tables_lds, 1 << 30);
         add(base.compact, base.wsum_reg ? 0 : (reg_sums_fit(cm.NW) ? 1 : 2), base.tables_lds, 1 << 30);
@@ -1121,6 +1167,22 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
         Shape h = base;
         make_state_hbm(h, pack_blob(cm, rt, r32, base.compact != 0).lay.bytes, 0);
         consider(h);
+        // act alone in HBM (x in LDS, word sums in VGPRs): the most resident waves, and the
+        // most up to 12 per CU (3 per SIMD leave the step loop its ~170 VGPRs without spills)
+        if (base.wsum_reg == 1) {
+            const uint32_t blob = pack_blob(cm, rt, r32, base.compact != 0).lay.bytes;
+            int prev = 0;
+            for (int below : {1 << 30, 13}) {
+                int waves = 0;
+                const int block = layout_block(c, cm.NW, 1, blob, base.tables_lds, &waves, below, 2);
+                if (!block || block == prev) continue;
+                prev = block;
+                Shape a = base;
+                a.state_hbm = 2;
+                a.block = block;
+                consider(a);
+            }
+        }
     }
     // step knobs on the best layout
     {
