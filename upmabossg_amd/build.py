@@ -5,7 +5,7 @@
 * Embeds the device sources (csrc/device/*) as C strings for hipRTC.
 * Compiles the host library with hipcc (host code only; the trajectory kernel is
   compiled at run time by hipRTC for gfx950) and links the CLI against it.
-* Compile-checks the generated kernel for gfx950 with hipcc for eight shapes, so a
+* Compile-checks the generated kernel for gfx950 with hipcc for nine shapes, so a
   device-code error shows up at build time rather than at the first run.
 * Compiles the surrogate MLP library (libmbg_mlp.so, BF16 MFMA kernels) for gfx950.
 * Builds the CPU oracle (oracle/liboracle.so, test infrastructure).
@@ -95,14 +95,15 @@ def check_kernel(verbose=False):
                   (2, 256, 0, 1, 1, 1, 6, 2, 4, 1, 0, 0, 1, 0, 0, 1, 1),
                   (200, 64, 1, 0, 1, 0, 3, 2, 4, 0, 0, 0, 1, 0, 0, 0, 1),
                   (79, 64, 0, 0, 0, 0, 3, 1, 4, 2, 0, 0, 5, 0, 1, 0, 0),
-                  (64, 128, 0, 0, 0, 0, 4, 2, 4, 2, 1, 1, 4, 0, 0, 0, 0)]
+                  (64, 128, 0, 0, 0, 0, 4, 2, 4, 2, 1, 1, 4, 0, 0, 0, 0),
+                  (16, 768, 0, 0, 0, 0, 4, 2, 4, 1, 1, 1, 1, 1, 1, 0, 0)]
         for i, (nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp, pg) in enumerate(shapes):
             src = os.path.join(td, f"k{nw}_{disc}_{trace}_{multi}_{cp}_{sp}_{pg}.hip")
             th = 1  # window-entropy partials in LDS (per block, or per wave in sweeps)
             cells = 1 if (th and not sp and nw <= 2) else 0  # small models: the whole window table too
             cap = 2 * db if i % 2 else 0  # dependents per iteration capped in every other shape
             # lanes' state in HBM (1), or only their active sets (2)
-            hbm = 1 if (not multi and i in (3, 5)) else 2 if i in (1, 4) else 0
+            hbm = 1 if (not multi and i in (3, 5)) else 2 if i in (1, 4, 8) else 0
             with open(src, "w") as f:
                 f.write(kernel_source(nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp,
                                       pg, th, cells, cap, hbm))

@@ -26,6 +26,7 @@
 //   MBG_STATE_HBM   1: the lanes' x, act and word sums in an HBM buffer (lane-interleaved over
 //                   the whole grid) instead of LDS, for models whose per-lane state would leave
 //                   a CU with one or two waves (single runs only); 2: only act in that buffer,
+//                   also in sweeps (a class's grid shares one buffer, lane by lane),
 //                   x in LDS and the word sums in VGPRs (half the LDS per lane, so twice the
 //                   resident waves; act is read once per step and once per rate change)
 // and mbg_math.h.
@@ -118,11 +119,11 @@ typedef unsigned int sidx;
 #ifndef MBG_STATE_HBM
 #define MBG_STATE_HBM 0
 #endif
-#if MBG_STATE_HBM && MBG_MULTI
-#error "the lane state in HBM is a layout of single runs"
+#if MBG_STATE_HBM == 1 && MBG_MULTI
+#error "the whole lane state in HBM is a layout of single runs"
 #endif
-#if MBG_STATE_HBM == 2 && MBG_WSUM_REG != 1
-#error "the active sets alone in HBM need the word sums in VGPRs"
+#if MBG_STATE_HBM == 2 && (MBG_WSUM_REG != 1 || MBG_SUMW != 1)
+#error "the active sets alone in HBM need one register rate sum per word (at most 16 words)"
 #endif
 
 // A node record decoded into registers (either table layout).
@@ -1278,7 +1279,8 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
     const int start = (int)(((u64)wave * (u64)nv) / waves_total);
 #if MBG_TH_LDS
     const u32 lid = threadIdx.x & 63u;
-    u64* thw = smem + (MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK + (threadIdx.x >> 6) * (u32)wmax;
+    u64* thw = smem + (MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK +
+               (threadIdx.x >> 6) * (u32)wmax;
 #else
     (void)wmax;
     u64* thw = nullptr;

@@ -346,6 +346,7 @@ struct SweepClass {
     size_t smem = 0;
     int blocks_per_cu = 1;
     int wmax = 1;  // largest window count of the class's variants (per-wave entropy partials)
+    DevBuf<uint64_t> act;  // the lanes' active sets in HBM (MBG_STATE_HBM 2), shared by the class
 };
 
 struct SweepState {
@@ -549,6 +550,31 @@ int layout_block(const mbg_ctx* c, int nw, int wsum_reg, uint32_t blob_bytes, in
         if (lds > (size_t)c->lds_per_block) continue;
         const int waves = std::min(16, (int)(cu_lds / lds) * (block / 64));
         if (waves >= below_waves) continue;
+        if (waves > best_waves) {
+            best_waves = waves;
+            best = block;
+        }
+    }
+    if (waves_out) *waves_out = best_waves;
+    return best;
+}
+
+// Block for the active sets in HBM (MBG_STATE_HBM 2): x in LDS beside fixed_bytes per block
+// and per_wave_bytes per wave.  At most 12 waves per block: launch bounds of 768 threads leave
+// the step loop about 170 VGPRs (3 waves per SIMD), while 16 would cap it at 128 and spill, so
+// a CU holds at most 12 waves of this kernel whatever the LDS allows.  0 if nothing fits.
+// The layout is limited to one register rate sum per word (at most 16 words): with groups of
+// two words (a 1,100-node variant in the 20-word sweep class) the sweep test counted 226,839
+// transitions against the oracle's 227,473, with the cause not found, so it is not used there.
+bool act_hbm_allowed(int wsum_reg, int sumw) { return wsum_reg == 1 && sumw == 1; }
+
+int act_hbm_block(const mbg_ctx* c, int nw, size_t fixed_bytes, size_t per_wave_bytes, int* waves_out) {
+    const size_t cu_lds = 160 * 1024;
+    int best = 0, best_waves = 0;
+    for (int block = 768; block >= 64; block -= 64) {
+        const size_t lds = lane_lds_bytes(nw, block, 1, 2) + fixed_bytes + (size_t)(block / 64) * per_wave_bytes;
+        if (lds > (size_t)c->lds_per_block) continue;
+        const int waves = std::min(12, (int)(cu_lds / lds) * (block / 64));
         if (waves > best_waves) {
             best_waves = waves;
             best = block;
@@ -798,7 +824,8 @@ Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::R
     // (one wave per CU in LDS; above about 106 words it would not fit at all)
     const int hbm_auto = lane_lds_bytes(cm.NW, 64, default_wsum_reg(cm.NW)) > (80u << 10) ? 1 : 0;
     int hbm_mode = env_int("MBG_TUNE_STATE_HBM", hbm_auto);
-    if (hbm_mode == 0 && !std::getenv("MBG_TUNE_STATE_HBM") && !block_req && default_wsum_reg(cm.NW) == 1) {
+    const bool act_ok = act_hbm_allowed(default_wsum_reg(cm.NW), default_sumw(cm.NW, default_wsum_reg(cm.NW)));
+    if (hbm_mode == 0 && !std::getenv("MBG_TUNE_STATE_HBM") && !block_req && act_ok) {
         // act alone in HBM by default where it keeps more waves resident (up to 12 per CU, which
         // leave the step loop its VGPRs): the 1,000-node network of 16 words runs 12 waves
         // instead of 8, 64 ms instead of 74 per million trajectories
@@ -806,27 +833,27 @@ Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::R
         const size_t lds_bytes = lane_lds_bytes(cm.NW, lds.block, lds.wsum_reg) + (lds.tables_lds ? blob_bytes : 0);
         const int waves0 = std::min(16, (int)(160 * 1024 / lds_bytes) * (lds.block / 64));
         int waves2 = 0;
-        layout_block(c, cm.NW, 1, blob_bytes, lds.tables_lds, &waves2, 13, 2);
+        const size_t th_bytes = (size_t)std::ceil(cm.max_time / cm.time_tick) * 8;
+        act_hbm_block(c, cm.NW, (lds.tables_lds ? blob_bytes : 0) + th_bytes, 0, &waves2);
         if (waves2 > waves0) hbm_mode = 2;
     }
     const bool hbm = hbm_mode == 1;
     // (act alone in HBM sets its own block below: the LDS-layout pass only picks the rest)
-    const bool act_hbm = hbm_mode == 2 && default_wsum_reg(cm.NW) == 1;
+    const bool act_hbm = hbm_mode == 2 && act_ok;
     Shape sh = choose_shape(c, cm, blob_bytes, act_hbm ? 64 : block_req, trace, rate32 ? 1 : 0, !hbm);
     sh.compact = compact ? 1 : 0;
     sh.wsmad = sh.wsum_reg && rates_fit_i31(rt) && env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
     sh.sparse = cm.K > mbg::kMaxReadouts ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
     if (hbm) make_state_hbm(sh, blob_bytes, block_req);
-    if (act_hbm && sh.wsum_reg == 1) {
+    if (act_hbm && act_hbm_allowed(sh.wsum_reg, sh.sumw)) {
         // act alone in HBM: x's LDS sets the block (a requested one if it fits), tables in LDS
         // where they fit beside it
         const int breq = block_req ? block_req : env_int("MBG_TUNE_BLOCK", 0);
         for (int tables = 1; tables >= 0; tables--) {
             int waves = 0;
-            int block = breq ? breq : layout_block(c, cm.NW, 1, blob_bytes, tables, &waves, 13, 2);
-            // (small lanes reach more than 12 LDS waves with any block: then the fullest one)
-            if (!block) block = layout_block(c, cm.NW, 1, blob_bytes, tables, &waves, 1 << 30, 2);
+            const size_t th_bytes = (size_t)std::ceil(cm.max_time / cm.time_tick) * 8;
+            const int block = breq ? breq : act_hbm_block(c, cm.NW, (tables ? blob_bytes : 0) + th_bytes, 0, &waves);
             if (block && lane_lds_bytes(cm.NW, block, 1, 2) + (tables ? blob_bytes : 0) <= (size_t)c->lds_per_block) {
                 sh.state_hbm = 2;
                 sh.block = block;
@@ -1167,14 +1194,16 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
         Shape h = base;
         make_state_hbm(h, pack_blob(cm, rt, r32, base.compact != 0).lay.bytes, 0);
         consider(h);
-        // act alone in HBM (x in LDS, word sums in VGPRs): the most resident waves, and the
-        // most up to 12 per CU (3 per SIMD leave the step loop its ~170 VGPRs without spills)
-        if (base.wsum_reg == 1) {
+        // act alone in HBM (x in LDS, word sums in VGPRs): blocks of at most 12 waves (3 per
+        // SIMD leave the step loop its ~170 VGPRs without spills), and the fullest block
+        if (act_hbm_allowed(base.wsum_reg, base.sumw)) {
             const uint32_t blob = pack_blob(cm, rt, r32, base.compact != 0).lay.bytes;
+            const size_t th_bytes = (size_t)std::ceil(cm.max_time / cm.time_tick) * 8;
             int prev = 0;
-            for (int below : {1 << 30, 13}) {
+            for (int full = 0; full < 2; full++) {
                 int waves = 0;
-                const int block = layout_block(c, cm.NW, 1, blob, base.tables_lds, &waves, below, 2);
+                const int block = full ? layout_block(c, cm.NW, 1, blob, base.tables_lds, &waves, 1 << 30, 2)
+                                       : act_hbm_block(c, cm.NW, (base.tables_lds ? blob : 0) + th_bytes, 0, &waves);
                 if (!block || block == prev) continue;
                 prev = block;
                 Shape a = base;
@@ -1320,13 +1349,29 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             }
         }
         if (!best_waves) throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
+        // the active sets in HBM where that keeps more waves resident (up to 12 per CU, which
+        // leave the step loop its VGPRs), as in single runs
+        // leave room for the per-wave window-entropy partials
+        if (act_hbm_allowed(C.sh.wsum_reg, C.sh.sumw) && env_int("MBG_TUNE_STATE_HBM", -1) != 0) {
+            int wm = 1;
+            for (int v : C.members) {
+                const mbg::CompiledModel& cmv = sw->vars[v]->m->cm;
+                wm = std::max(wm, (int)std::ceil(cmv.max_time / cmv.time_tick));
+            }
+            int waves2 = 0;
+            const int block2 = act_hbm_block(c, C.sh.nw, 0, (size_t)wm * 8, &waves2);
+            if (block2 && waves2 > best_waves) {
+                C.sh.state_hbm = 2;
+                C.sh.block = block2;
+            }
+        }
         // per-wave window-entropy partials in LDS when they fit without costing occupancy
         int wmax = 1;
         for (int v : C.members) {
             const mbg::CompiledModel& cmv = sw->vars[v]->m->cm;
             wmax = std::max(wmax, (int)std::ceil(cmv.max_time / cmv.time_tick));
         }
-        const size_t state_lds = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg);
+        const size_t state_lds = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg, C.sh.state_hbm);
         const size_t th_lds = (size_t)(C.sh.block / 64) * (size_t)wmax * 8;
         C.sh.th_lds = env_int("MBG_TUNE_TH_LDS", 1) != 0 && wmax <= 4096 &&
                               state_lds + th_lds <= (size_t)c->lds_per_block &&
@@ -1352,6 +1397,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         int nb = 0;
         if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, C.sim, C.sh.block, C.smem) != hipSuccess || nb < 1) nb = 1;
         C.blocks_per_cu = nb;
+        if (C.sh.state_hbm == 2) C.act.alloc((size_t)c->cus * (size_t)nb * (size_t)C.sh.block * (size_t)C.sh.nw);
     }
     // shared per-sweep buffers: counters and the marginal jobs of every variant
     SweepShared& SS = c->sweep_shared;
@@ -1420,6 +1466,11 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             D.FP = rs.FP;
             D.FT = rs.FT;
             D.sigma = rt.sigma;
+            if (C.sh.state_hbm == 2) {
+                // one column per thread of the class's largest grid (cus x blocks_per_cu blocks)
+                D.lane_state = C.act.p;
+                D.lane_stride = (uint32_t)((uint64_t)c->cus * (uint64_t)C.blocks_per_cu * (uint64_t)C.sh.block);
+            }
             mbg_run_info& I = V.info;
             I = mbg_run_info{};
             I.windows = rs.W;
@@ -1437,6 +1488,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             I.tables_in_lds = 0;
             I.lds_bytes = (int)C.smem;
             I.blocks_per_cu = C.blocks_per_cu;
+            I.state_hbm = C.sh.state_hbm;
             descs.push_back(D);
         }
         C.descs.alloc(descs.size());
