@@ -112,9 +112,10 @@ typedef struct mbg_run_params {
 mbg_status mbg_run(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p);
 
 /* Measured launch shape (no reference counterpart; the reference compiles one fixed launch
- * configuration): times candidate shapes on `sample` trajectories of the model (0: up to
- * 262,144), by coordinate descent over table placement, compact tables, register or LDS rate
- * sums (or the lanes' whole state in HBM), block size, dependents per round and per iteration,
+ * configuration): times candidate shapes on `sample` trajectories of the model (0: the
+ * model's count, at most four per lane of a full chip, at least 262,144), by coordinate descent
+ * over table placement, compact tables, register or LDS rate sums (or the lanes' whole state,
+ * or their active sets, in HBM), block size, dependents per round and per iteration,
  * search width and multiply-add sum updates, and
  * keeps the fastest for later runs of models of the same structure (explicit blocks in
  * mbg_run_params override it).  Results never depend on the shape.  Discards the last run. */

@@ -1127,7 +1127,12 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
                                                      m->mult_down.empty() ? nullptr : m->mult_down.data(),
                                                      m->istate.empty() ? nullptr : m->istate.data());
     const bool r32 = rates_fit_u32(rt), r31 = rates_fit_i31(rt), cfit = compact_fits(cm);
-    const uint64_t n = sample ? sample : std::min<uint64_t>((uint64_t)cm.sample_count, 262144);
+    // default sample: about four trajectories per lane of a full chip (1,024 lanes per CU at
+    // most), so that the run's tail (lanes out of work while others finish) weighs as it does in
+    // a run of millions; with one or two per lane the tail decided the dependents cap
+    const uint64_t n = sample ? sample
+                              : std::min<uint64_t>((uint64_t)cm.sample_count,
+                                                   std::max<uint64_t>(262144, 4ull * (uint64_t)c->cus * 1024ull));
     mbg_run_params rp{};
     rp.n_trajectories = n;
     rp.count = n;
@@ -1150,9 +1155,12 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
     Shape best = base;
     const double base_ms = measure(base);
     double best_ms = base_ms;
+    // a candidate replaces the best only when it is more than 1 % faster: smaller differences
+    // are within the sample's run-to-run spread, and the untuned default is the better-studied
+    // shape on the full run
     auto consider = [&](const Shape& sh) {
         const double ms = measure(sh);
-        if (ms < best_ms) {
+        if (ms < best_ms * 0.99) {
             best_ms = ms;
             best = sh;
         }
