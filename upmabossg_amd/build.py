@@ -30,7 +30,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-W
 HOST_LIBS = ["-pthread", "-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
 KERNEL_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-HOST_SOURCES = ["front.cpp", "model.cpp", "output.cpp", "runtime.cpp", "variants.cpp"]
+HOST_SOURCES = ["front.cpp", "model.cpp", "output.cpp", "pack.cpp", "runtime.cpp", "variants.cpp"]
 
 
 def _run(cmd, verbose):

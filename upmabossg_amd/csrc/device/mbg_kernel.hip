@@ -64,7 +64,19 @@ typedef unsigned short u16;
 typedef unsigned long long u64;
 
 // Pointers into memory that no kernel writes while it runs (the model tables).
+#ifndef CONST_AS
 #define CONST_AS __attribute__((address_space(4)))
+#endif
+// A value the compiler must treat as unknown from here on (an empty asm that "writes" it), in
+// a vector register (V) or a scalar one (S): keeps a computation in the form it is written in.
+#ifndef MBG_OPAQUE_V
+#define MBG_OPAQUE_V(x) asm volatile("" : "+v"(x))
+#define MBG_OPAQUE_S(x) asm volatile("" : "+s"(x))
+#endif
+// The kernel's dynamic LDS, sized at launch.
+#ifndef MBG_DYN_LDS
+#define MBG_DYN_LDS(name) extern __shared__ u64 name[]
+#endif
 
 struct mbg_node {
     u32 in_off;               // first input in the inputs table
@@ -122,7 +134,10 @@ typedef unsigned int sidx;
 #if MBG_STATE_HBM == 1 && MBG_MULTI
 #error "the whole lane state in HBM is a layout of single runs"
 #endif
-#if MBG_STATE_HBM == 2 && (MBG_WSUM_REG != 1 || MBG_SUMW != 1)
+// The active sets alone in HBM with sum groups of several words: correct in the host emulation
+// of this source, but one device run of that layout miscounted (cause not identified), so the
+// runtime never compiles it; the host emulation defines MBG_ACT_GROUPS_PROBE to keep checking it.
+#if MBG_STATE_HBM == 2 && (MBG_WSUM_REG != 1 || MBG_SUMW != 1) && !defined(MBG_ACT_GROUPS_PROBE)
 #error "the active sets alone in HBM need one register rate sum per word (at most 16 words)"
 #endif
 
@@ -304,11 +319,11 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
 #define WS_GADD(g, d)                                                      \
     do {                                                                   \
         u32 oh_ = 1u << (g);                                               \
-        asm volatile("" : "+v"(oh_));                                      \
+        MBG_OPAQUE_V(oh_);                                                 \
         const long long d_ = (long long)(int)(u32)(d);                     \
         _Pragma("unroll") for (int i_ = 0; i_ < NSUM; i_++) {              \
             u32 f_ = (oh_ >> i_) & 1u;                                     \
-            asm volatile("" : "+v"(f_)); /* keep the multiply-add */       \
+            MBG_OPAQUE_V(f_); /* keep the multiply-add */                  \
             ws[i_] = (u64)((long long)ws[i_] + (long long)(int)f_ * d_);   \
         }                                                                  \
     } while (0)
@@ -944,7 +959,8 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 // wave-uniform): hoisted out of the loop, its 20 round keys would not fit in
                 // SGPRs and would come back through VALU lane reads
                 u32 k0 = __builtin_amdgcn_readfirstlane(key0), k1 = __builtin_amdgcn_readfirstlane(key1);
-                asm volatile("" : "+s"(k0), "+s"(k1));
+                MBG_OPAQUE_S(k0);
+                MBG_OPAQUE_S(k1);
                 mbg_philox((u32)(k >> 1), 1u, (u32)id, D.variant_id, k0, k1, d4);
             }
             const u32 dr1 = (k & 1ull) ? d4[2] : d4[0];
@@ -1212,7 +1228,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 
 #if !MBG_MULTI
 extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D) {
-    extern __shared__ u64 smem[];
+    MBG_DYN_LDS(smem);
 #if MBG_TABLES_LDS
     // stage the model tables in LDS once per (persistent) block
     {
@@ -1273,7 +1289,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
 // With MBG_TH_LDS each wave sums its current variant's window entropies in its own LDS slice
 // of wmax cells (after the lanes' state) and flushes them when it moves to the next variant.
 extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const mbg_desc* descs, int nv, int wmax) {
-    extern __shared__ u64 smem[];
+    MBG_DYN_LDS(smem);
     const u32 waves_total = gridDim.x * (MBG_BLOCK / 64);
     const u32 wave = blockIdx.x * (MBG_BLOCK / 64) + (threadIdx.x >> 6);
     const int start = (int)(((u64)wave * (u64)nv) / waves_total);
