@@ -129,9 +129,13 @@ def surrogate_bench():
         "plain_points_per_s": sc["plain_points_per_s"],
         "screened_points_per_s": sc["screened_points_per_s"],
         "speedup_at_0.01": sc["speedup_at_gate"],
-        "matched_rms_trajectories": sc["matched_rms_trajectories"],
-        "surrogate_points_per_s": sc["surrogate_points_per_s"],
-        "speedup_at_matched_rms": sc["speedup_at_matched_rms"],
+        "target": ">= 20x the points/s of plain batched simulation at <= 0.01 max abs probability error",
+        "target_met": bool(sc["speedup_at_gate"] >= 20.0),
+        # not the target metric: plain simulation at the trajectory count whose RMS error matches
+        # the surrogate's (its own max error is far above 0.01)
+        "secondary": {"matched_rms_trajectories": sc["matched_rms_trajectories"],
+                      "surrogate_points_per_s": sc["surrogate_points_per_s"],
+                      "speedup_at_matched_rms": sc["speedup_at_matched_rms"]},
         "roofline": {"bound": "mfma", "achieved": sc["inference_tflops"], "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": sc["inference_tflops"] / 2500.0},
     }
@@ -158,7 +162,7 @@ def ring_bench(ctx, sizes=(100, 500), cpu=True):
              "transitions": int(i.transitions), "kernel_ms": i.kernel_ms, "wall_ms": wall * 1e3,
              "transitions_per_s": i.transitions / wall}
         if cpu:
-            cores = min(16, os.cpu_count() or 1)
+            cores = host_cpu()[2]
             cnt = 16384 if n <= 100 else 2048  # a few seconds of CPU work
             r = OracleModel(b, c).run(count=cnt, n_trajectories=1000000, threads=cores, record_fixed=False)
             e["cpu_transitions_per_s"] = r.flips / r.seconds
@@ -167,15 +171,39 @@ def ring_bench(ctx, sizes=(100, 500), cpu=True):
     return out
 
 
+def host_cpu():
+    """CPU model name (from /proc/cpuinfo), logical CPUs the OS reports and the threads used: the
+    CPU share of a one-GPU box is 16 threads, whatever the machine's total."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = logical
+    return model, logical, max(1, min(16, avail))
+
+
 def cpu_baseline(net, count=4096):
     from oracle.oracle import OracleModel
 
-    cores = min(16, os.cpu_count() or 1)
+    model, logical, cores = host_cpu()
     o = OracleModel(net.bnd(), net.cfg())
     r = o.run(count=count, n_trajectories=net.sample_count, threads=cores, record_fixed=True)
     return {"value": r.flips / r.seconds, "unit": "transitions/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "logical_cpus_reported": logical,
+            "per_thread": r.flips / r.seconds / cores,
+            "algorithm": "the reference algorithm in the CPU oracle: every node's rate recomputed each step by "
+                         "interpreting its expression tree, sequential sum, linear flip scan; one trajectory per thread",
             "sample": f"trajectory ids 0-{count - 1} of the same workload ({r.flips} transitions, "
-                      f"{r.seconds:.1f} s)"}
+                      f"{r.seconds:.1f} s, {cores} threads: the CPU share of a one-GPU box)"}
 
 
 def main():
