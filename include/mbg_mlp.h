@@ -55,6 +55,17 @@ mbg_status mbg_mlp_predict(mbg_mlp* m, const float* x, size_t n, float* y, doubl
 mbg_status mbg_mlp_ensemble_predict(mbg_mlp* const* members, int count, const float* x, size_t n, float* mean,
                                     float* spread, double* compute_ms);
 
+/* The same ensemble prediction without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  x [n][n_in], mean [n][n_out] and spread [n] (may be NULL) are
+ * DEVICE buffers of the members' device.  The members' outputs are staged in a fixed 64 MiB
+ * device array of the library (16,777,216 floats) in chunks of rows, so members x n_out x 64
+ * must not exceed 16,777,216 (MBG_ERR_ARG otherwise); n itself is unbounded.  Calls overlapping
+ * on one device share that array and race: run them one at a time per device.  Synchronous on
+ * the first member's stream; the results equal mbg_mlp_ensemble_predict's. */
+mbg_status mbg_mlp_ensemble_predict_noalloc(mbg_mlp* const* members, int count, const float* x, size_t n, float* mean,
+                                            float* spread);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(256) void ensemble_reduce(const float* __restrict__
         if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) spread[row] = red[0];
+    if (threadIdx.x == 0 && spread) spread[row] = red[0];
 }
 
 struct Error : std::runtime_error {
@@ -273,6 +275,32 @@ struct Buf {
 
 unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
+// Scratch of mbg_mlp_ensemble_predict_noalloc: the members' outputs of one chunk of rows,
+// [member][row][output].  64 MiB per device, held by every device the library runs on until the
+// process ends.  Host code uses only the address hipGetSymbolAddress gives for it on each
+// device (scratch_addr), never the array's name.
+constexpr size_t kScratchFloats = 16u << 20;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+std::mutex g_scratch_mu;
+std::map<int, float*> g_scratch_addr;  // device -> that device's copy of scratch
+
+// Looks the array up once per device, with that device current (from mbg_mlp_create).
+void init_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr.count(device)) return;
+    void* p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw Error(4, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_addr[device] = static_cast<float*>(p);
+}
+
+float* scratch_addr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    const auto it = g_scratch_addr.find(device);
+    return it == g_scratch_addr.end() ? nullptr : it->second;
+}
+
 }  // namespace
 
 struct mbg_mlp {
@@ -298,37 +326,42 @@ struct mbg_mlp {
 
     template <int BM, int BN>
     void launch_gemm(const bf16* A, long lda, bool ak, const bf16* B, long ldb, bool bk, float* C, long ldc, long M,
-                     long N, long K, bool acc) {
+                     long N, long K, bool acc, hipStream_t st) {
         dim3 grid((unsigned)(N / BN), (unsigned)(M / BM));
-        if (ak && bk) gemm_bf16<true, true, BM, BN><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
-        else if (ak && !bk) gemm_bf16<true, false, BM, BN><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
-        else if (!ak && bk) gemm_bf16<false, true, BM, BN><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
-        else gemm_bf16<false, false, BM, BN><<<grid, 256, 0, s>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
+        if (ak && bk) gemm_bf16<true, true, BM, BN><<<grid, 256, 0, st>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
+        else if (ak && !bk) gemm_bf16<true, false, BM, BN><<<grid, 256, 0, st>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
+        else if (!ak && bk) gemm_bf16<false, true, BM, BN><<<grid, 256, 0, st>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
+        else gemm_bf16<false, false, BM, BN><<<grid, 256, 0, st>>>(A, lda, B, ldb, C, ldc, (int)K, acc);
     }
 
+    // C = A B on stream st (the network's own stream when null)
     void gemm(const bf16* A, long lda, bool ak, const bf16* B, long ldb, bool bk, float* C, long ldc, long M, long N,
-              long K, bool acc) {
+              long K, bool acc, hipStream_t st = nullptr) {
         if (M % 64 || N % 64 || K % TK) throw Error(3, "GEMM dimensions must be multiples of 64x64x32");
+        if (!st) st = s;
         // the largest tile that still gives every CU a block
         const bool bm = M % 128 == 0 && (M / 128) * (N / 64) >= 256;
         const bool bn = N % 128 == 0 && (M / (bm ? 128 : 64)) * (N / 128) >= 256;
-        if (bm && bn) launch_gemm<128, 128>(A, lda, ak, B, ldb, bk, C, ldc, M, N, K, acc);
-        else if (bm) launch_gemm<128, 64>(A, lda, ak, B, ldb, bk, C, ldc, M, N, K, acc);
-        else launch_gemm<64, 64>(A, lda, ak, B, ldb, bk, C, ldc, M, N, K, acc);
+        if (bm && bn) launch_gemm<128, 128>(A, lda, ak, B, ldb, bk, C, ldc, M, N, K, acc, st);
+        else if (bm) launch_gemm<128, 64>(A, lda, ak, B, ldb, bk, C, ldc, M, N, K, acc, st);
+        else launch_gemm<64, 64>(A, lda, ak, B, ldb, bk, C, ldc, M, N, K, acc, st);
         check(hipGetLastError(), "gemm launch");
     }
 
-    // Forward pass of `rows` rows already in H[0]; targets in Y when with_loss.
-    void forward(long rows, bool with_loss) {
+    // Forward pass of `rows` rows already in H[0]; targets in Y when with_loss.  On stream st
+    // (the network's own stream when null).
+    void forward(long rows, bool with_loss, hipStream_t st = nullptr) {
+        if (!st) st = s;
         const int L = (int)din.size();
         for (int l = 0; l < L; l++) {
-            gemm(H[l].p, din[l], true, wb.p + woff[l], din[l], true, Cm[l].p, dout[l], rows, dout[l], din[l], false);
+            gemm(H[l].p, din[l], true, wb.p + woff[l], din[l], true, Cm[l].p, dout[l], rows, dout[l], din[l], false, st);
             if (l < L - 1) {
-                bias_gelu<<<blocks(rows * dout[l]), 256, 0, s>>>(Cm[l].p, w.p + boff[l], Z[l].p, H[l + 1].p, rows, dout[l]);
+                bias_gelu<<<blocks(rows * dout[l]), 256, 0, st>>>(Cm[l].p, w.p + boff[l], Z[l].p, H[l + 1].p, rows,
+                                                                 dout[l]);
             } else {
-                sigmoid_mse<<<blocks(rows * dout[l]), 256, 0, s>>>(Cm[l].p, w.p + boff[l], with_loss ? Y.p : nullptr, Yhat.p,
-                                                                   dZ.p, dZb.p, sq.p, rows, dout[l],
-                                                                   1.f / (float)(rows * dout[l]));
+                sigmoid_mse<<<blocks(rows * dout[l]), 256, 0, st>>>(Cm[l].p, w.p + boff[l], with_loss ? Y.p : nullptr,
+                                                                    Yhat.p, dZ.p, dZb.p, sq.p, rows, dout[l],
+                                                                    1.f / (float)(rows * dout[l]));
             }
         }
     }
@@ -423,6 +456,7 @@ mbg_status mbg_mlp_create(int device, int n_in, int hidden, int hidden_layers, i
     m->n_out = n_out;
     m->max_batch = max_batch;
     const mbg_status st = guarded(m, [&] {
+        init_scratch(device);
         check(hipStreamCreate(&m->s), "hipStreamCreate");
         m->din.push_back(m->in_p);
         for (int l = 0; l < hidden_layers; l++) {
@@ -610,6 +644,54 @@ mbg_status mbg_mlp_ensemble_predict(mbg_mlp* const* members, int count, const fl
         check(hipEventElapsedTime(&t, ev[0], ev[count + 1]), "hipEventElapsedTime");
         for (auto& e : ev) (void)hipEventDestroy(e);
         if (compute_ms) *compute_ms = t;
+    });
+}
+
+mbg_status mbg_mlp_ensemble_predict_noalloc(mbg_mlp* const* members, int count, const float* x, size_t n, float* mean,
+                                            float* spread) {
+    if (!members || count <= 0 || !members[0] || !x || !mean || n == 0) return 3;
+    mbg_mlp* m0 = members[0];
+    long chunk = (long)m0->max_batch;
+    for (int k = 0; k < count; k++) {
+        const mbg_mlp* mk = members[k];
+        if (!mk || mk->device != m0->device || mk->n_in != m0->n_in || mk->n_out != m0->n_out) {
+            m0->err = "ensemble members must share the device and the input and output widths";
+            return 3;
+        }
+        chunk = std::min(chunk, (long)mk->max_batch);
+    }
+    // rows per chunk: every member's batch and the scratch's capacity, in whole blocks of 64
+    const size_t per_row = (size_t)count * (size_t)m0->n_out;
+    const long cap_rows = (long)(kScratchFloats / per_row) / 64 * 64;
+    if (cap_rows < 64) {
+        m0->err = "ensemble too large for the no-allocation scratch: members x outputs x 64 rows must fit " +
+                  std::to_string(kScratchFloats) + " floats (" + std::to_string(count) + " x " +
+                  std::to_string(m0->n_out) + " given)";
+        return 3;
+    }
+    chunk = std::min(chunk, cap_rows);
+    return guarded(m0, [&] {
+        float* P = scratch_addr(m0->device);
+        if (!P) throw Error(3, "no scratch address for this device (no network was created on it)");
+        const int n_in = m0->n_in, n_out = m0->n_out;
+        hipStream_t st = m0->s;
+        for (size_t r0 = 0; r0 < n; r0 += (size_t)chunk) {
+            const long rows = (long)std::min<size_t>((size_t)chunk, n - r0);
+            const long padded = (rows + 63) / 64 * 64;
+            for (int k = 0; k < count; k++) {
+                mbg_mlp* mk = members[k];
+                pad_bf16<<<blocks(padded * mk->in_p), 256, 0, st>>>(x + r0 * n_in, mk->H[0].p, rows, padded, n_in,
+                                                                   mk->in_p);
+                mk->forward(padded, false, st);
+                check(hipMemcpyAsync(P + (size_t)k * rows * n_out, mk->Yhat.p, (size_t)rows * n_out * 4,
+                                     hipMemcpyDeviceToDevice, st),
+                      "hipMemcpyAsync");
+            }
+            ensemble_reduce<<<(unsigned)rows, 256, 0, st>>>(P, count, rows, n_out, mean + r0 * n_out,
+                                                            spread ? spread + r0 : nullptr);
+            check(hipGetLastError(), "ensemble_reduce launch");
+        }
+        check(hipStreamSynchronize(st), "ensemble predict (no allocation)");
     });
 }
 

@@ -16,7 +16,9 @@ _LIB = None
 
 
 class MlpError(RuntimeError):
-    pass
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
 
 
 def lib():
@@ -41,6 +43,7 @@ def lib():
         "mbg_mlp_predict": (C.c_int, [vp, f32p, C.c_size_t, f32p, C.POINTER(C.c_double)]),
         "mbg_mlp_ensemble_predict": (C.c_int, [C.POINTER(vp), C.c_int, f32p, C.c_size_t, f32p, f32p,
                                                C.POINTER(C.c_double)]),
+        "mbg_mlp_ensemble_predict_noalloc": (C.c_int, [C.POINTER(vp), C.c_int, vp, C.c_size_t, vp, vp]),
     }
     for name, (rt, args) in sig.items():
         f = getattr(L, name)
@@ -150,6 +153,17 @@ def ensemble_predict(members, x):
     if st != 0:
         raise MlpError(lib().mbg_mlp_last_error(members[0]._h).decode())
     return mean, spread, ms.value
+
+
+def ensemble_predict_noalloc(members, x_dev, n, mean_dev, spread_dev=None):
+    """The ensemble prediction on device buffers (addresses as ints: x [n][n_in] float32, mean
+    [n][n_out], spread [n] or None), with no device allocation (mbg_mlp_ensemble_predict_noalloc)."""
+    handles = (C.c_void_p * len(members))(*[m._h.value for m in members])
+    st = lib().mbg_mlp_ensemble_predict_noalloc(handles, len(members), C.c_void_p(x_dev), n, C.c_void_p(mean_dev),
+                                                C.c_void_p(spread_dev) if spread_dev else None)
+    if st != 0:
+        raise MlpError(lib().mbg_mlp_last_error(members[0]._h).decode(), st)
+    return st
 
 
 def _bench(argv=None):
