@@ -325,16 +325,103 @@ def test_large_full_size_properties(gpu_ctx):
     b = gpu_ctx.run(m)
     assert np.array_equal(a.prob_raw, b.prob_raw) and np.array_equal(a.th_raw, b.th_raw)
     assert np.array_equal(a.final_states, b.final_states) and a.transitions == b.transitions
-    # 4-sigma agreement of the readout marginals with an oracle sample of 4,096 trajectories
-    o = OracleModel(net.bnd(), net.cfg())
-    r = o.run(count=4096, threads=16)
+
+
+def _z_scores(pg, ng, po, no):
+    """Two-sample binomial z-scores of probabilities pg (from ng trajectories) and po (from no),
+    with the pooled p and a variance floor of one trajectory's weight in the smaller sample, so
+    that a state seen in a single trajectory is not a many-sigma event.  For window-averaged
+    probabilities (time fractions) the binomial variance is an upper bound."""
+    p = (pg * ng + po * no) / (ng + no)
+    var = np.maximum(p * (1 - p), 1.0 / min(ng, no)) * (1.0 / ng + 1.0 / no)
+    return (pg - po) / np.sqrt(var)
+
+
+def _family_threshold(m, alpha=1e-3):
+    """The per-entry |z| limit: 4 (the 4-sigma rule), raised to the Bonferroni level when m
+    entries are tested together (family-wise false-alarm rate alpha), where that is above 4."""
+    from scipy.stats import norm
+
+    return max(4.0, float(norm.isf(alpha / (2 * max(1, m)))))
+
+
+def _stat_report(name, z):
+    m = z.size
+    thr = _family_threshold(m)
+    beyond4 = int(np.sum(np.abs(z) > 4.0))
+    _flush(f"{name}: {m} entries, max |z| {np.max(np.abs(z)):.2f}, {beyond4} beyond 4 sigma "
+           f"(expected under the null at most {m * 6.334e-5:.2f}), limit {thr:.2f}")
+    return thr
+
+
+def test_large_full_size_statistics_independent_seed(gpu_ctx):
+    """1,048,576 GPU trajectories of the 1,000-node network (seed 1) against an independent
+    oracle sample (8,192 trajectories, seed 2): every window x projected-state entry where
+    either side has mass, every readout marginal and every final state, within 4 sigma (the
+    Bonferroni level for the number of entries where that is higher)."""
+    net = synth.config_large()
+    m = Model(net.bnd(), net.cfg())
+    _flush("case: large full size, seed 1, against an oracle sample with seed 2")
+    a = gpu_ctx.run(m, seed=1)
+    no = 8192
+    r = OracleModel(net.bnd(), net.cfg()).run(count=no, n_trajectories=no, seed=2, threads=16, record_fixed=False)
+    ng = a.simulated
+    tick = float(np.float32(net.time_tick))
+    po = r.prob.astype(np.float64) / 2.0 ** r.prob_exponent / (no * tick)
+    pg = a.probtraj
+    mass = (pg > 0) | (po > 0)
+    z = _z_scores(pg[mass], ng, po[mass], no)
+    assert np.all(np.abs(z) <= _stat_report("window x state entries", z))
     K = a.readouts
-    S_ = 1 << K
-    bits = (np.arange(S_)[:, None] >> np.arange(K)[None, :]) & 1
-    pg = a.probtraj @ bits
-    po = (r.prob.astype(np.float64) / 2.0 ** r.prob_exponent / (4096 * 0.5)) @ bits
-    sig = np.sqrt(np.maximum(pg * (1 - pg), 1e-4) / 4096)
-    assert np.all(np.abs(pg - po) <= 4.5 * sig)
+    bits = (np.arange(1 << K)[:, None] >> np.arange(K)[None, :]) & 1
+    zm = _z_scores(pg @ bits, ng, po @ bits, no)
+    assert np.all(np.abs(zm) <= _stat_report("readout marginals", zm))
+    fg, fo = a.final_states / ng, r.final / no
+    fm = (fg > 0) | (fo > 0)
+    zf = _z_scores(fg[fm], ng, fo[fm], no)
+    assert np.all(np.abs(zf) <= _stat_report("final states", zf))
+
+
+def test_fixed_point_heavy_full_size_statistics_and_dedup(gpu_ctx, monkeypatch):
+    """262,144 GPU trajectories of the 5,000-node fixed-point-heavy network (seed 1) against an
+    independent oracle sample (384 trajectories, seed 2): the fraction reaching a fixed point,
+    the final states and the readout marginals within 4 sigma, and every fixed point of the
+    sample at its GPU probability within 4 sigma.  The device deduplication of the fixed-point
+    records gives the same distinct states and counts as sorting every record on the host."""
+    net = synth.config_fixed_heavy()
+    m = Model(net.bnd(), net.cfg())
+    _flush("case: fixed-point-heavy full size, device deduplication")
+    a = gpu_ctx.run(m, seed=1)
+    ng = a.simulated
+    assert ng == 262144 and int(a.fixed_counts.sum()) == a.fixed_point_trajectories
+    _flush("case: fixed-point-heavy full size, host deduplication")
+    monkeypatch.setenv("MBG_TUNE_FP_HOST", "1")
+    h = gpu_ctx.run(m, seed=1)
+    monkeypatch.delenv("MBG_TUNE_FP_HOST")
+    assert np.array_equal(a.fixed_states, h.fixed_states) and np.array_equal(a.fixed_counts, h.fixed_counts)
+    _flush(f"  {a.fixed_point_trajectories} fixed-point trajectories, {len(a.fixed_counts)} distinct fixed points")
+    no = 384
+    r = OracleModel(net.bnd(), net.cfg()).run(count=no, n_trajectories=no, seed=2, threads=16)
+    fo_frac = float(r.fixed_counts.sum()) / no
+    zfp = _z_scores(np.array([a.fixed_point_trajectories / ng]), ng, np.array([fo_frac]), no)
+    assert abs(zfp[0]) <= _stat_report("fixed-point fraction", zfp)
+    fg, fo = a.final_states / ng, r.final / no
+    fm = (fg > 0) | (fo > 0)
+    zf = _z_scores(fg[fm], ng, fo[fm], no)
+    assert np.all(np.abs(zf) <= _stat_report("final states", zf))
+    tick = float(np.float32(net.time_tick))
+    K = a.readouts
+    bits = (np.arange(1 << K)[:, None] >> np.arange(K)[None, :]) & 1
+    po = r.prob.astype(np.float64) / 2.0 ** r.prob_exponent / (no * tick)
+    zm = _z_scores(a.probtraj @ bits, ng, po @ bits, no)
+    assert np.all(np.abs(zm) <= _stat_report("readout marginals", zm))
+    # the sample's fixed points at their GPU probabilities (0 when the GPU run never reached one)
+    gpu_fp = {a.fixed_states[i].tobytes(): int(a.fixed_counts[i]) for i in range(len(a.fixed_counts))}
+    pgf = np.array([gpu_fp.get(st.tobytes(), 0) / ng for st in r.fixed_states])
+    pof = r.fixed_counts.astype(np.float64) / no
+    if len(pof):
+        zz = _z_scores(pgf, ng, pof, no)
+        assert np.all(np.abs(zz) <= _stat_report("fixed points of the sample", zz))
 
 
 @pytest.mark.parametrize("n,readouts,discrete,count", [(48, 24, False, 4000), (300, 40, False, 2000),

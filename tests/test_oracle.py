@@ -169,6 +169,46 @@ def test_planted_selection_error_is_caught():
     assert not np.all(ctmc.within_sigmas(got, probs, n))
 
 
+@pytest.mark.parametrize("k", range(2))
+def test_variant_overrides_match_exact_ctmc_of_the_rewritten_network(k):
+    """orc_set_variant (rate_up / rate_down multipliers, a knockout as rate_up x 0 with istate 0,
+    an istate override) gives the exact CTMC solution of the network written out with the
+    scaled rates and initial probabilities, and not that of the unmodified network; every
+    sweep and knockout parity test of the GPU compares against these overrides."""
+    import copy
+
+    net = _tiny_networks()[k]
+    rng = np.random.default_rng(20 + k)
+    mu = np.exp(rng.uniform(np.log(0.25), np.log(4.0), net.n)).astype(np.float32)
+    md = np.exp(rng.uniform(np.log(0.25), np.log(4.0), net.n)).astype(np.float32)
+    ist = rng.uniform(0, 1, net.n).astype(np.float32)
+    ko = 1  # knockout: never switched on, starts off
+    mu[ko] = 0.0
+    ist[ko] = 0.0
+    m = O.OracleModel(net.bnd(), net.cfg())
+    m.set_variant(mult_up=mu, mult_down=md, istate=ist)
+    n = net.sample_count
+    r = m.run(count=n, threads=8)
+    rew = copy.deepcopy(net)
+    rew.up = (net.up * mu).astype(np.float32)
+    rew.down = (net.down * md).astype(np.float32)
+    rew.istate = ist
+    probs, _, final, _ = ctmc.continuous_solution(rew, r.windows)
+    got = r.prob.astype(np.float64) / 2.0 ** r.prob_exponent / (n * float(np.float32(net.time_tick)))
+    assert np.all(ctmc.within_sigmas(got, probs, n)), np.max(np.abs(got - probs))
+    assert np.all(ctmc.within_sigmas(r.final / n, final, n))
+    # the knocked-out node is off in every final state that projects it
+    if ko in net.readouts:
+        bit = net.readouts.index(ko)
+        assert all(r.final[s] == 0 for s in range(len(r.final)) if (s >> bit) & 1)
+    base, _, _, _ = ctmc.continuous_solution(net, r.windows)
+    assert not np.all(ctmc.within_sigmas(got, base, n)), "the overrides changed nothing"
+    # the rewritten network itself, run without overrides, gives the same tables bit for bit
+    # when its rates are the same FP32 products
+    r2 = O.OracleModel(rew.bnd(), rew.cfg()).run(count=n, threads=8)
+    assert np.array_equal(r2.prob, r.prob) and np.array_equal(r2.final, r.final) and r2.flips == r.flips
+
+
 def _discrete_exact(net_rates, p0, proj, K, nsteps):
     """Jump chain with the reference's discrete-time window rule (a fixed point is counted only in
     the window where it is reached, src/jit_kernels/window_average_small.cu:101-136)."""

@@ -558,7 +558,8 @@ void collect_fixed_points(mbg_ctx* c, const Module& mod, const uint64_t* d_fp, u
     states.clear();
     counts.clear();
     if (nfp == 0) return;
-    if (nfp <= 4096) {
+    // MBG_TUNE_FP_HOST=1: every record to the host and sorted there (the check of the device path)
+    if (nfp <= 4096 || env_int("MBG_TUNE_FP_HOST", 0) != 0) {
         std::vector<uint64_t> fps((size_t)nfp * stride);
         hip_check(hipMemcpy(fps.data(), d_fp, fps.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
         unique_fixed_points(fps, stride, keep, states, counts);
