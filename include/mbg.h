@@ -104,6 +104,10 @@ typedef struct mbg_run_params {
     int compat_batch_resume; /* 1: reproduce the reference's 100-record launches (no flip after
                                 every 99th step, a fresh waiting time for the same state next;
                                 src/jit_kernels/simulation.cu:104-147); 0: uninterrupted Gillespie */
+    int error_columns;       /* 1: also accumulate what the ErrorTH / ErrorProba columns need (the
+                                reference writes 0 there, src/statistics/window_average_small.cpp:
+                                211,235,245): per trajectory and window its time in each projected
+                                state and its TH x time, squared and summed (exact integers) */
 } mbg_run_params;
 
 /* Simulates and accumulates the statistics (replaces simulation_runner::run_simulation,
@@ -139,6 +143,9 @@ typedef struct mbg_run_info {
     float time_tick, max_time;
     int block, tables_in_lds, lds_bytes, blocks_per_cu;  /* launch shape chosen for the run */
     int state_hbm;             /* lanes' state in HBM: 0 none (LDS), 1 all of it, 2 the active sets */
+    int error_columns;         /* the run accumulated the error columns' sums of squares */
+    int sq_exponent, th_sq_exponent; /* their units: 2^-sq_exponent of a squared window fraction
+                                (count when discrete), 2^-th_sq_exponent of a squared window TH */
 } mbg_run_info;
 mbg_status mbg_get_run_info(const mbg_ctx* c, mbg_run_info* out);
 
@@ -167,6 +174,19 @@ mbg_status mbg_get_fixed_points(const mbg_ctx* c, uint64_t* states, uint64_t* co
  * node indices, final time (FP64 bits), status (1 finished, 2 fixed point). */
 mbg_status mbg_get_summaries(const mbg_ctx* c, uint64_t* out, size_t count);
 
+/* Error columns of the last run (run with error_columns = 1), as MaBoSS prints them beside each
+ * probability: the standard error of the mean over trajectories of each trajectory's fraction of
+ * window w spent in projected state s (its step count when discrete), err_prob[W * 2^K], and of
+ * each trajectory's window-average TH, err_th[W]: sqrt((E[f^2] - E[f]^2) / (n - 1)), 0 when n < 2.
+ * Replaces the literal zeros of src/statistics/window_average_small.cpp:211,235,245.  Either
+ * pointer may be NULL; MBG_ERR_NOT_BUILT if the run did not accumulate them. */
+mbg_status mbg_get_probtraj_errors(const mbg_ctx* c, double* err_prob, double* err_th);
+/* The exact sums behind them: prob_sq[W * 2^K] and th_sq[W] (units in mbg_run_info). */
+mbg_status mbg_get_probtraj_squares_raw(const mbg_ctx* c, uint64_t* prob_sq, uint64_t* th_sq);
+/* Above 20 readouts: the sums of squares of the non-zero window cells, in the order of
+ * mbg_get_probtraj_cells. */
+mbg_status mbg_get_probtraj_cell_squares(const mbg_ctx* c, uint64_t* squares, size_t cap, size_t* n);
+
 /* Readout marginals of the last run: out[W * K] = probability that readout j is ON, averaged
  * over window w (the window table summed over the projected states with bit j set). */
 mbg_status mbg_get_marginals(const mbg_ctx* c, double* out);
@@ -189,6 +209,10 @@ typedef struct mbg_variant {
  * reference's CLI once per variant; README.md:33 lists mutant analysis as future work).
  * Synchronous.  The variants' tables stay on the device until the next run. */
 mbg_status mbg_run_variants(mbg_ctx* c, const mbg_variant* variants, int count);
+/* The same with flags: MBG_SWEEP_ERROR_COLUMNS also accumulates every variant's error-column
+ * sums (as mbg_run_params.error_columns does for one run). */
+#define MBG_SWEEP_ERROR_COLUMNS 1
+mbg_status mbg_run_variants_ex(mbg_ctx* c, const mbg_variant* variants, int count, int flags);
 
 typedef struct mbg_sweep_info {
     int variants, classes;
@@ -200,6 +224,8 @@ mbg_status mbg_get_sweep_info(const mbg_ctx* c, mbg_sweep_info* out);
 /* Per-variant results of the last sweep (same layouts as the single-run getters). */
 mbg_status mbg_get_variant_info(const mbg_ctx* c, int v, mbg_run_info* out);
 mbg_status mbg_get_variant_probtraj_raw(const mbg_ctx* c, int v, uint64_t* prob, uint64_t* th);
+/* Variant v's error-column sums of squares (a sweep run with MBG_SWEEP_ERROR_COLUMNS). */
+mbg_status mbg_get_variant_probtraj_squares_raw(const mbg_ctx* c, int v, uint64_t* prob_sq, uint64_t* th_sq);
 mbg_status mbg_get_variant_marginals(const mbg_ctx* c, int v, double* out);
 mbg_status mbg_get_variant_final_states(const mbg_ctx* c, int v, uint64_t* counts);
 mbg_status mbg_get_variant_fixed_points(const mbg_ctx* c, int v, uint64_t* states, uint64_t* counts, size_t cap,

@@ -810,10 +810,26 @@ static void build(orc_model& m) {
 // ---------------------------------------------------------------------------
 // Simulation.
 // ---------------------------------------------------------------------------
+// Square of one trajectory's total v over a window (its time in one projected state, or its
+// TH x time), in the fixed-point units of a sum of squares: rint((v * scale)^2 * 2^F), where
+// scale turns v into the trajectory's fraction of the window (or its window-average TH).  The
+// sums of these squares give the standard errors of the window averages (ErrorProba, ErrorTH).
+static inline uint64_t sq_units(uint64_t v, double scale, int F) {
+    const double x = (double)v * scale;
+    return (uint64_t)nearbyint(ldexp(x * x, F));
+}
+
 struct Tables {
     int W = 0, K = 0, FP = 0, FT = 0;
     uint64_t S = 0;
     std::vector<uint64_t> prob, th, fin;
+    // sums over trajectories of the squared per-trajectory window totals: prob2 [W][S] (cell
+    // maps above 20 readouts), th2 [W]; exponents FQ, FQT and the scales of sq_units
+    int FQ = 0, FQT = 0;
+    double sq_scale = 1.0, sq_scale_th = 1.0;
+    std::vector<uint64_t> prob2, th2;
+    std::vector<std::map<std::pair<uint32_t, uint64_t>, uint64_t>> cell2_maps;
+    std::vector<uint64_t> cv2;
     // more than 20 readouts: per-thread maps of the non-zero cells ((window, state) -> sum) and
     // final states, merged after the run into sorted lists (window-major, states ascending)
     bool sparse = false;
@@ -827,6 +843,13 @@ struct Tables {
             __atomic_fetch_add(&prob[(uint64_t)w * S + s], v, __ATOMIC_RELAXED);
         } else if (v) {
             cell_maps[tix][{(uint32_t)w, s}] += v;
+        }
+    }
+    void cell2(int tix, int w, uint64_t s, uint64_t q) {
+        if (!sparse) {
+            __atomic_fetch_add(&prob2[(uint64_t)w * S + s], q, __ATOMIC_RELAXED);
+        } else if (q) {
+            cell2_maps[tix][{(uint32_t)w, s}] += q;
         }
     }
     void final_state(int tix, uint64_t s) {
@@ -908,6 +931,32 @@ static void simulate_range(const orc_model& m, const orc_params& p, uint64_t b, 
         double t = 0.0;
         uint64_t k = 0, nflip = 0, h = 0xcbf29ce484222325ull;
         bool fixed_pt = false;
+        // this trajectory's totals over the current window: time (or steps) per projected state
+        // visited, revisits merged, and TH x time; their squares go to prob2 / th2 when the
+        // window ends
+        std::vector<std::pair<uint64_t, uint64_t>> wcells;
+        int cw = -1;
+        uint64_t wth = 0;
+        auto close_window = [&]() {
+            if (cw < 0) return;
+            for (const auto& c : wcells) T.cell2(tix, cw, c.first, sq_units(c.second, T.sq_scale, T.FQ));
+            if (wth) atomic_add(&T.th2[cw], sq_units(wth, T.sq_scale_th, T.FQT));
+            wcells.clear();
+            wth = 0;
+        };
+        auto window_add = [&](int w, uint64_t s, uint64_t v, uint64_t thv) {
+            if (w != cw) {
+                close_window();
+                cw = w;
+            }
+            if (v) {
+                size_t j = 0;
+                while (j < wcells.size() && wcells[j].first != s) j++;
+                if (j == wcells.size()) wcells.push_back({s, v});
+                else wcells[j].second += v;
+            }
+            wth += thv;
+        };
         for (;;) {
             uint64_t total = 0;
             for (int i = 0; i < N; i++) {
@@ -941,8 +990,10 @@ static void simulate_range(const orc_model& m, const orc_params& p, uint64_t b, 
             if (discrete) {
                 int w = (int)floor(t * inv_tick + 0.5);
                 if (w > T.W - 1) w = T.W - 1;
+                const uint64_t thv = (uint64_t)nearbyint(ldexp(th, T.FT));
                 T.cell(tix, w, s, 1);
-                atomic_add(&T.th[w], (uint64_t)nearbyint(ldexp(th, T.FT)));
+                atomic_add(&T.th[w], thv);
+                window_add(w, s, 1, thv);
             } else {
                 int w = (int)floor(t * inv_tick);
                 double bgn = t;
@@ -951,8 +1002,11 @@ static void simulate_range(const orc_model& m, const orc_params& p, uint64_t b, 
                     double en = t1 < we ? t1 : we;
                     double sl = en - bgn;
                     int ww = w > T.W - 1 ? T.W - 1 : w;
-                    T.cell(tix, ww, s, (uint64_t)nearbyint(ldexp(sl, T.FP)));
-                    atomic_add(&T.th[ww], (uint64_t)nearbyint(ldexp(th * sl, T.FT)));
+                    const uint64_t v = (uint64_t)nearbyint(ldexp(sl, T.FP));
+                    const uint64_t thv = (uint64_t)nearbyint(ldexp(th * sl, T.FT));
+                    T.cell(tix, ww, s, v);
+                    atomic_add(&T.th[ww], thv);
+                    window_add(ww, s, v, thv);
                     w++;
                     bgn = en;
                 }
@@ -983,6 +1037,7 @@ static void simulate_range(const orc_model& m, const orc_params& p, uint64_t b, 
             nflip++;
             k++;
         }
+        close_window();
         uint64_t s = 0;
         for (size_t j = 0; j < m.readouts.size(); j++) s |= ((x[m.readouts[j] >> 6] >> (m.readouts[j] & 63)) & 1ull) << j;
         T.final_state(tix, s);
@@ -1095,9 +1150,17 @@ orc_result* orc_run(orc_model* m, const orc_params* p, char* err, size_t errlen)
             T.FP = 60 - eq;
             T.FT = 57 - eq;
         }
+        // squares: a fraction of a window is at most 1, a window-average TH below 64 (6 bits of
+        // entropy at most), so n squares stay below 2^62
+        T.FQ = 62 - ceil_log2_u64(n);
+        T.FQT = 56 - ceil_log2_u64(n);
+        T.sq_scale = discrete ? 1.0 : ldexp(1.0, -T.FP) / (double)tick_f;
+        T.sq_scale_th = discrete ? ldexp(1.0, -T.FT) : ldexp(1.0, -T.FT) / (double)tick_f;
         T.prob.assign((uint64_t)T.W * T.S, 0);
         T.th.assign(T.W, 0);
         T.fin.assign(T.S, 0);
+        T.prob2.assign((uint64_t)T.W * T.S, 0);
+        T.th2.assign(T.W, 0);
         res->NW = m->NW;
         int nt = std::max(1, p->threads);
         uint64_t cnt = p->count;
@@ -1105,6 +1168,7 @@ orc_result* orc_run(orc_model* m, const orc_params* p, char* err, size_t errlen)
         std::vector<std::vector<std::vector<uint64_t>>> fx(nt);
         if (T.sparse) {
             T.cell_maps.resize(nt);
+            T.cell2_maps.resize(nt);
             T.fin_maps.resize(nt);
         }
         std::vector<uint64_t> fl(nt, 0);
@@ -1128,12 +1192,18 @@ orc_result* orc_run(orc_model* m, const orc_params* p, char* err, size_t errlen)
                 for (auto& kv : mp) cells[kv.first] += kv.second;
             for (auto& mp : T.fin_maps)
                 for (auto& kv : mp) fins[kv.first] += kv.second;
+            std::map<std::pair<uint32_t, uint64_t>, uint64_t> cells2;
+            for (auto& mp : T.cell2_maps)
+                for (auto& kv : mp) cells2[kv.first] += kv.second;
             T.cell_maps.clear();
+            T.cell2_maps.clear();
             T.fin_maps.clear();
             for (auto& kv : cells) {
                 T.cw.push_back(kv.first.first);
                 T.cs.push_back(kv.first.second);
                 T.cv.push_back(kv.second);
+                const auto q = cells2.find(kv.first);
+                T.cv2.push_back(q == cells2.end() ? 0 : q->second);
             }
             for (auto& kv : fins) {
                 T.fs.push_back(kv.first);
@@ -1145,6 +1215,7 @@ orc_result* orc_run(orc_model* m, const orc_params* p, char* err, size_t errlen)
                     T.cw.push_back((uint32_t)(i / T.S));
                     T.cs.push_back(i % T.S);
                     T.cv.push_back(T.prob[i]);
+                    T.cv2.push_back(T.prob2[i]);
                 }
             for (uint64_t i = 0; i < T.fin.size(); i++)
                 if (T.fin[i]) {
@@ -1184,6 +1255,11 @@ int orc_th_exponent(const orc_result* r) { return r->t.FT; }
 const uint64_t* orc_prob_table(const orc_result* r) { return r->t.prob.data(); }
 const uint64_t* orc_th_table(const orc_result* r) { return r->t.th.data(); }
 const uint64_t* orc_final_table(const orc_result* r) { return r->t.fin.data(); }
+const uint64_t* orc_prob_sq_table(const orc_result* r) { return r->t.prob2.data(); }
+const uint64_t* orc_th_sq_table(const orc_result* r) { return r->t.th2.data(); }
+int orc_sq_exponent(const orc_result* r) { return r->t.FQ; }
+int orc_th_sq_exponent(const orc_result* r) { return r->t.FQT; }
+const uint64_t* orc_cell_squares(const orc_result* r) { return r->t.cv2.data(); }
 uint64_t orc_cell_count(const orc_result* r) { return r->t.cv.size(); }
 const uint32_t* orc_cell_windows(const orc_result* r) { return r->t.cw.data(); }
 const uint64_t* orc_cell_states(const orc_result* r) { return r->t.cs.data(); }

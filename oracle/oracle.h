@@ -101,6 +101,14 @@ uint64_t orc_cell_count(const orc_result* r);
 const uint32_t* orc_cell_windows(const orc_result* r);
 const uint64_t* orc_cell_states(const orc_result* r);
 const uint64_t* orc_cell_values(const orc_result* r);
+/* Error columns: sums over trajectories of squared per-trajectory window totals, W*S cells
+ * (window-major) and W TH cells, in units of 2^-FQ (resp. 2^-FQT) of the squared fraction of
+ * the window (resp. squared window-average TH); orc_cell_squares follows orc_cell_values. */
+const uint64_t* orc_prob_sq_table(const orc_result* r);
+const uint64_t* orc_th_sq_table(const orc_result* r);
+int orc_sq_exponent(const orc_result* r);
+int orc_th_sq_exponent(const orc_result* r);
+const uint64_t* orc_cell_squares(const orc_result* r);
 uint64_t orc_final_cell_count(const orc_result* r);
 const uint64_t* orc_final_cell_states(const orc_result* r);
 const uint64_t* orc_final_cell_counts(const orc_result* r);

@@ -75,10 +75,12 @@ def lib():
     L.orc_run.restype = vp
     L.orc_run.argtypes = [vp, C.POINTER(_Params), C.c_char_p, C.c_size_t]
     L.orc_result_free.argtypes = [vp]
-    for n in ["orc_windows", "orc_readouts", "orc_prob_exponent", "orc_th_exponent", "orc_state_words"]:
+    for n in ["orc_windows", "orc_readouts", "orc_prob_exponent", "orc_th_exponent", "orc_state_words",
+              "orc_sq_exponent", "orc_th_sq_exponent"]:
         getattr(L, n).restype = C.c_int
         getattr(L, n).argtypes = [vp]
-    for n in ["orc_prob_table", "orc_th_table", "orc_final_table", "orc_fixed_states", "orc_fixed_counts"]:
+    for n in ["orc_prob_table", "orc_th_table", "orc_final_table", "orc_fixed_states", "orc_fixed_counts",
+              "orc_prob_sq_table", "orc_th_sq_table", "orc_cell_squares"]:
         getattr(L, n).restype = u64p
         getattr(L, n).argtypes = [vp]
     for n in ["orc_cell_states", "orc_cell_values", "orc_final_cell_states", "orc_final_cell_counts"]:
@@ -129,6 +131,13 @@ class OracleResult:
     summaries: np.ndarray | None
     cells: tuple = None  # non-zero cells (window uint32, state uint64, value uint64), window-major
     final_cells: tuple = None  # non-zero final states (state, count), ascending
+    # error columns: sums of squared per-trajectory window totals (units 2^-sq_exponent of the
+    # squared window fraction, 2^-th_sq_exponent of the squared window-average TH)
+    prob_sq: np.ndarray | None = None  # (W, S) uint64; None above 20 readouts
+    th_sq: np.ndarray | None = None  # (W,) uint64
+    cell_squares: np.ndarray | None = None  # aligned with cells
+    sq_exponent: int = 0
+    th_sq_exponent: int = 0
 
 
 class OracleModel:
@@ -226,11 +235,13 @@ class OracleModel:
             W = L.orc_windows(r)
             K = L.orc_readouts(r)
             th = np.ctypeslib.as_array(L.orc_th_table(r), shape=(W,)).copy()
+            th_sq = np.ctypeslib.as_array(L.orc_th_sq_table(r), shape=(W,)).copy()
             if K > 20:
-                prob = fin = None
+                prob = fin = prob_sq = None
             else:
                 S = 1 << K
                 prob = np.ctypeslib.as_array(L.orc_prob_table(r), shape=(W * S,)).copy().reshape(W, S)
+                prob_sq = np.ctypeslib.as_array(L.orc_prob_sq_table(r), shape=(W * S,)).copy().reshape(W, S)
                 fin = np.ctypeslib.as_array(L.orc_final_table(r), shape=(S,)).copy()
 
             def arr(ptr, n, dt):
@@ -255,6 +266,8 @@ class OracleModel:
                 buf = (C.c_char * (count * SUMMARY_DTYPE.itemsize)).from_address(sp)
                 summ = np.frombuffer(bytes(buf), dtype=SUMMARY_DTYPE).copy()
             return OracleResult(W, K, L.orc_prob_exponent(r), L.orc_th_exponent(r), prob, th, fin,
-                                L.orc_total_flips(r), L.orc_elapsed_seconds(r), fs, fc, summ, cells, final_cells)
+                                L.orc_total_flips(r), L.orc_elapsed_seconds(r), fs, fc, summ, cells, final_cells,
+                                prob_sq, th_sq, arr(L.orc_cell_squares(r), nc, np.uint64), L.orc_sq_exponent(r),
+                                L.orc_th_sq_exponent(r))
         finally:
             L.orc_result_free(r)

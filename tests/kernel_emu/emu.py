@@ -92,8 +92,9 @@ def _planner() -> str:
     return exe
 
 
-def _job(td, variants, grid=1, compat=0, cell_cap=1 << 16, stride_extra=0):
-    lines = [f"grid {grid}", f"compat {compat}", f"cell_cap {cell_cap}", f"stride_extra {stride_extra}"]
+def _job(td, variants, grid=1, compat=0, cell_cap=1 << 16, stride_extra=0, err_cap=8):
+    lines = [f"grid {grid}", f"compat {compat}", f"cell_cap {cell_cap}", f"stride_extra {stride_extra}",
+             f"err_cap {err_cap}"]
     for i, v in enumerate(variants):
         bnd, cfg = os.path.join(td, f"m{i}.bnd"), os.path.join(td, f"m{i}.cfg")
         with open(bnd, "w") as f:
@@ -158,11 +159,11 @@ def _f32(path, v):
 
 
 def run(exe: str, variants, grid: int = 1, compat: int = 0, cell_cap: int = 1 << 16, stride_extra: int = 0,
-        timeout: int = 900):
+        err_cap: int = 8, timeout: int = 900):
     """variants: list of dicts with bnd, cfg (text), n, count, first_id, seed, vid and optional
     mu, md, istate arrays.  Returns one dict of raw tables per variant."""
     with tempfile.TemporaryDirectory() as td:
-        job = _job(td, variants, grid, compat, cell_cap, stride_extra)
+        job = _job(td, variants, grid, compat, cell_cap, stride_extra, err_cap)
         env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0")
         r = subprocess.run([exe, job, td], capture_output=True, text=True, timeout=timeout, env=env)
         if r.returncode:
@@ -176,7 +177,9 @@ def run(exe: str, variants, grid: int = 1, compat: int = 0, cell_cap: int = 1 <<
             fp = rd("fp").reshape(-1, nw)
             out.append(dict(flips=int(info[0]), nfp=int(info[1]), W=int(info[2]), K=int(info[3]), FP=int(info[4]),
                             FT=int(info[5]), cell_miss=int(info[7]), prob=rd("prob"), th=rd("th"), fin=rd("fin"),
-                            fp_records=fp, summ=rd("summ").reshape(-1, 4), cells=rd("cells"), cellv=rd("cellv")))
+                            fp_records=fp, summ=rd("summ").reshape(-1, 4), cells=rd("cells"), cellv=rd("cellv"),
+                            err_miss=int(info[8]), FQ=int(info[9]), FQT=int(info[10]), prob2=rd("prob2"),
+                            th2=rd("th2"), cellv2=rd("cellv2")))
         return out
 
 

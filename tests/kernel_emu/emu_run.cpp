@@ -7,7 +7,7 @@
 // and blob packer (front.cpp, model.cpp, pack.cpp).
 //
 // Usage: emu_run JOB OUTDIR
-//   JOB lines:  grid G | compat 0/1 | cell_cap C | stride_extra E |
+//   JOB lines:  grid G | compat 0/1 | cell_cap C | stride_extra E | err_cap M |
 //               variant BND CFG N COUNT FIRST_ID SEED VARIANT_ID MU MD IS
 //   (MU / MD / IS: files of N float32 values, or "-"; one variant line for a single run)
 // Writes OUTDIR/v<i>_<name>.u64 (prob, th, fin, info, fp, summ, cells, cellv) per variant.
@@ -62,7 +62,7 @@ struct Variant {
     uint64_t n = 0, count = 0, first_id = 0, seed = 0;
     uint32_t vid = 0;
     // emulated device buffers, each exactly sized (bounds-checked under AddressSanitizer)
-    std::vector<u64> stage, prob, th, fin, fp, summ, counters, cells, cellv, cell_miss;
+    std::vector<u64> stage, prob, th, fin, fp, summ, counters, cells, cellv, cell_miss, prob2, th2, cellv2;
     std::vector<float> istate;
     mbg_desc D{};
 };
@@ -81,7 +81,7 @@ int main(int argc, char** argv) {
         std::vector<std::unique_ptr<Variant>> vars;
         unsigned grid = 1;
         int compat = 0;
-        uint64_t cell_cap = 1 << 16, stride_extra = 0;
+        uint64_t cell_cap = 1 << 16, stride_extra = 0, err_cap = 8;
         std::string line;
         while (std::getline(job, line)) {
             std::istringstream ls(line);
@@ -91,6 +91,7 @@ int main(int argc, char** argv) {
             else if (key == "compat") ls >> compat;
             else if (key == "cell_cap") ls >> cell_cap;
             else if (key == "stride_extra") ls >> stride_extra;
+            else if (key == "err_cap") ls >> err_cap;
             else if (key == "variant") {
                 auto V = std::make_unique<Variant>();
                 std::string bnd, cfg, mu, md, is;
@@ -116,6 +117,12 @@ int main(int argc, char** argv) {
 #if MBG_STATE_HBM
         lane_state.assign((size_t)stride * (MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW,
                           0x5A5A5A5A5A5A5A5Aull);
+#endif
+        // error columns: the lanes' window lists (one column per grid thread) and their overflow count
+        std::vector<u64> err_s, err_v, err_miss(1, 0);
+#if MBG_ERRORS
+        err_s.assign((size_t)err_cap * (grid * (uint64_t)MBG_BLOCK), 0x3C3C3C3C3C3C3C3Cull);
+        err_v.assign((size_t)err_cap * (grid * (uint64_t)MBG_BLOCK), 0x3C3C3C3C3C3C3C3Cull);
 #endif
         int wmax = 1;
         for (auto& Vp : vars) {
@@ -144,6 +151,11 @@ int main(int argc, char** argv) {
             V.cells.assign(cell_cap, ~0ull);
             V.cellv.assign(cell_cap, 0);
             V.cell_miss.assign(1, 0);
+#endif
+#if MBG_ERRORS
+            V.prob2.assign((size_t)V.rs.W * S, 0);
+            V.th2.assign((size_t)V.rs.W, 0);
+            if (MBG_SPARSE) V.cellv2.assign(cell_cap, 0);
 #endif
             mbg_desc& D = V.D;
             D.blob = reinterpret_cast<const char*>(V.blob.words.data());
@@ -186,7 +198,22 @@ int main(int argc, char** argv) {
 #endif
             D.lane_state = lane_state.empty() ? nullptr : lane_state.data();
             D.lane_stride = (u32)stride;
+#if MBG_ERRORS
+            D.prob2 = V.prob2.data();
+            D.th2 = V.th2.data();
+            D.cell_vals2 = V.cellv2.empty() ? nullptr : V.cellv2.data();
+            D.err_s = err_s.data();
+            D.err_v = err_v.data();
+            D.err_miss = err_miss.data();
+            D.sq_scale = V.rs.sq_scale;
+            D.sq_scale_th = V.rs.sq_scale_th;
+            D.FQ = V.rs.FQ;
+            D.FQT = V.rs.FQT;
+            D.err_cap = (u32)err_cap;
+            D.err_stride = (u32)(grid * (uint64_t)MBG_BLOCK);
+#endif
         }
+        const size_t th_cells = MBG_ERRORS ? 2 : 1;  // window entropies, then their squares
         const size_t lane_words =
             MBG_STATE_HBM == 1 ? 0 : (size_t)(MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
 #if !MBG_MULTI
@@ -204,7 +231,7 @@ int main(int argc, char** argv) {
             emu::launch({(unsigned)((V.count + 255) / 256), 1, 1}, {256, 1, 1}, 0, false, [&] { mbg_init_readouts(D); });
         }
         size_t smem_words = (MBG_TABLES_LDS ? V.blob.lay.bytes / 8 : 0) + lane_words;
-        if (MBG_TH_LDS) smem_words += (size_t)V.rs.W;
+        if (MBG_TH_LDS) smem_words += th_cells * (size_t)V.rs.W;
         if (MBG_CELLS_LDS) smem_words += ((size_t)V.rs.W + 1) * ((size_t)1 << V.cm.K);
         emu::launch({grid, 1, 1}, {MBG_BLOCK, 1, 1}, smem_words * 8, true, [&] { mbg_simulate(D); });
 #else
@@ -216,7 +243,7 @@ int main(int argc, char** argv) {
         }
         const mbg_desc* dp = descs.data();
         emu::launch({(unsigned)bx, (unsigned)descs.size(), 1}, {256, 1, 1}, 0, false, [&] { mbg_init_rates_multi(dp); });
-        const size_t smem_words = lane_words + (MBG_TH_LDS ? (size_t)(MBG_BLOCK / 64) * (size_t)wmax : 0);
+        const size_t smem_words = lane_words + (MBG_TH_LDS ? (size_t)(MBG_BLOCK / 64) * th_cells * (size_t)wmax : 0);
         const int nv = (int)descs.size();
         emu::launch({grid, 1, 1}, {MBG_BLOCK, 1, 1}, smem_words * 8, true,
                     [&] { mbg_simulate_multi(dp, nv, wmax); });
@@ -226,8 +253,9 @@ int main(int argc, char** argv) {
             const std::string p = out + "/v" + std::to_string(v) + "_";
             const uint64_t nfp = X.counters[0];
             if (nfp > X.count) throw std::runtime_error("fixed-point count beyond the trajectories");
-            const uint64_t info[8] = {X.counters[1], nfp, (uint64_t)X.rs.W, (uint64_t)X.cm.K, (uint64_t)X.rs.FP,
-                                      (uint64_t)X.rs.FT, (uint64_t)MBG_NW, X.cell_miss.empty() ? 0 : X.cell_miss[0]};
+            const uint64_t info[11] = {X.counters[1], nfp, (uint64_t)X.rs.W, (uint64_t)X.cm.K, (uint64_t)X.rs.FP,
+                                       (uint64_t)X.rs.FT, (uint64_t)MBG_NW, X.cell_miss.empty() ? 0 : X.cell_miss[0],
+                                       err_miss[0], (uint64_t)X.rs.FQ, (uint64_t)X.rs.FQT};
             dump(p + "info.u64", info, sizeof(info));
             dump(p + "prob.u64", X.prob.data(), X.prob.size() * 8);
             dump(p + "th.u64", X.th.data(), X.th.size() * 8);
@@ -236,6 +264,9 @@ int main(int argc, char** argv) {
             dump(p + "summ.u64", X.summ.data(), MBG_TRACE ? X.summ.size() * 8 : 0);
             dump(p + "cells.u64", X.cells.data(), X.cells.size() * 8);
             dump(p + "cellv.u64", X.cellv.data(), X.cellv.size() * 8);
+            dump(p + "prob2.u64", X.prob2.data(), X.prob2.size() * 8);
+            dump(p + "th2.u64", X.th2.data(), X.th2.size() * 8);
+            dump(p + "cellv2.u64", X.cellv2.data(), X.cellv2.size() * 8);
         }
     } catch (const std::exception& e) {
         std::cerr << "emu_run: " << e.what() << "\n";

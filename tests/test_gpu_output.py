@@ -12,6 +12,11 @@ The formatter below restates the reference's writers:
 with values printed as FP32 through the default ostream format (%g, 6 significant digits).
 Normalisation is this build's: cells are exact fixed-point sums, P = cell 2^-fp / (n tick)
 (counts / n in discrete time), TH = th 2^-ft / n (/ tick), evaluated in FP64 and printed as FP32.
+The columns the reference leaves at 0 are filled by default (MBG_ERROR_COLUMNS=0 keeps the
+zeros): ErrorProba / ErrorTH = sqrt((sum of squares 2^-e / n - mean^2) / (n - 1)), the standard
+error of the mean over trajectories of each one's window fraction (window-average TH), from
+the oracle's exact sums of squares; HD=0 = the window's total probability (the language has no
+reference state, so every state is at Hamming distance 0).
 """
 import ctypes
 import math
@@ -37,8 +42,16 @@ def g(v):
     return "%g" % float(f32(v))
 
 
+def _se(n, mean, raw_sq, e):
+    if n < 2:
+        return 0.0
+    var = math.ldexp(float(raw_sq), -e) / n - mean * mean
+    return math.sqrt(var / (n - 1)) if var > 0 else 0.0
+
+
 class Expected:
-    def __init__(self, bnd, cfg, variant=None, n=None, seed=None):
+    def __init__(self, bnd, cfg, variant=None, n=None, seed=None, errors=True):
+        self.errors = errors
         o = OracleModel(bnd, cfg)
         if variant is not None:
             o.set_variant(*variant)
@@ -58,39 +71,53 @@ class Expected:
         on = [self.names[i] for i in range(len(self.names)) if (int(words[i >> 6]) >> (i & 63)) & 1]
         return " -- ".join(on) if on else "<nil>"
 
-    def prob(self, raw):
+    def prob_d(self, raw):
         v = float(raw)
         if self.discrete:
-            return f32(v / self.n)
-        return f32(math.ldexp(v, -self.r.prob_exponent) / (self.n * float(self.tick)))
+            return v / self.n
+        return math.ldexp(v, -self.r.prob_exponent) / (self.n * float(self.tick))
 
-    def th(self, w):
+    def prob(self, raw):
+        return f32(self.prob_d(raw))
+
+    def th_d(self, w):
         v = math.ldexp(float(self.r.th[w]), -self.r.th_exponent) / self.n
         if not self.discrete:
             v /= float(self.tick)
-        return f32(v)
+        return v
+
+    def th(self, w):
+        return f32(self.th_d(w))
 
     def row(self, w):
-        # the oracle's non-zero cells (window-major, states ascending): any number of readouts
+        # the oracle's non-zero cells (window-major, states ascending): any number of readouts;
+        # (state, P, ErrorProba) per state, the entropy and the total probability (HD=0)
         cw, cs, cv = self.r.cells
         lo, hi = np.searchsorted(cw, w, "left"), np.searchsorted(cw, w, "right")
-        probs, ent = [], f32(0)
-        for s, raw in zip(cs[lo:hi], cv[lo:hi]):
+        probs, ent, total = [], f32(0), 0.0
+        for s, raw, sq in zip(cs[lo:hi], cv[lo:hi], self.r.cell_squares[lo:hi]):
             s = int(s)
-            p = self.prob(raw)
+            pd = self.prob_d(raw)
+            p = f32(pd)
             if p == 0:
                 continue
             ent = f32(ent + f32(f32(-_LIBM.log2f(p)) * p))
-            probs.append((s, p))
-        return probs, ent
+            total += pd
+            probs.append((s, p, f32(_se(self.n, pd, sq, self.r.sq_exponent)) if self.errors else 0))
+        return probs, ent, total
 
     def probtraj_csv(self):
         rows = [self.row(w) for w in range(self.r.windows)]
-        out = "Time\tTH\tErrorTH\tH\tHD=0" + "\tState\tProba\tErrorProba" * max(len(p) for p, _ in rows) + "\n"
-        for w, (probs, ent) in enumerate(rows):
-            out += "\t".join([g(f32(w) * self.tick), g(self.th(w)), "0", g(ent), "0"])
-            for s, p in probs:
-                out += f"\t{self.proj(s)}\t{g(p)}\t0"
+        out = "Time\tTH\tErrorTH\tH\tHD=0" + "\tState\tProba\tErrorProba" * max(len(p) for p, _, _ in rows) + "\n"
+        for w, (probs, ent, total) in enumerate(rows):
+            if self.errors:
+                err_th = g(f32(_se(self.n, self.th_d(w), self.r.th_sq[w], self.r.th_sq_exponent)))
+                hd0 = g(f32(total))
+            else:
+                err_th = hd0 = "0"
+            out += "\t".join([g(f32(w) * self.tick), g(self.th(w)), err_th, g(ent), hd0])
+            for s, p, e in probs:
+                out += f"\t{self.proj(s)}\t{g(p)}\t{g(e)}"
             out += "\n"
         return out
 
@@ -112,10 +139,10 @@ class Expected:
         for i, c in enumerate(self.r.fixed_counts):
             out += f"{g(f32(c) / n)} {self.full(self.r.fixed_states[i])}\n"
         for w in range(self.r.windows):
-            probs, ent = self.row(w)
+            probs, ent, _ = self.row(w)
             out += f"window ({g(f32(w) * self.tick)}, {g(f32(w + 1) * self.tick)}]\n"
             out += f"entropy: {g(ent)}\ntransition entropy: {g(self.th(w))}\n"
-            for s, p in probs:
+            for s, p, _ in probs:
                 out += f"{g(p)} {self.proj(s)}\n"
         return out
 
@@ -147,19 +174,21 @@ def _readouts24():
     return net.bnd(), net.cfg()
 
 
-@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("name", sorted(CASES) + ["metastasis_no_error_columns"])
 def test_cli_output_bytes(tmp_path, name):
-    bnd, cfg = CASES[name]()
+    errors = not name.endswith("_no_error_columns")
+    bnd, cfg = CASES[name.replace("_no_error_columns", "")]()
     bp, cp = tmp_path / "m.bnd", tmp_path / "m.cfg"
     bp.write_text(bnd)
     cp.write_text(cfg)
     print(f"case: CLI {name}", flush=True)
+    env = dict(os.environ, MBG_ERROR_COLUMNS="1" if errors else "0")
     run = subprocess.run([CLI_PATH, "-o", str(tmp_path / "out"), str(bp), str(cp)], capture_output=True, text=True,
-                         timeout=120)
+                         timeout=120, env=env)
     assert run.returncode == 0, run.stderr
-    text = subprocess.run([CLI_PATH, str(bp), str(cp)], capture_output=True, text=True, timeout=120)
+    text = subprocess.run([CLI_PATH, str(bp), str(cp)], capture_output=True, text=True, timeout=120, env=env)
     assert text.returncode == 0, text.stderr
-    want = Expected(bnd, cfg)
+    want = Expected(bnd, cfg, errors=errors)
     assert (tmp_path / "out_probtraj.csv").read_text() == want.probtraj_csv()
     assert (tmp_path / "out_fp.csv").read_text() == want.fp_csv()
     assert text.stdout == want.text()

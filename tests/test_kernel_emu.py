@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import OracleModel
+from tests.conftest import model_text
 from tests.kernel_emu import emu
 from upmabossg_amd import synth
 
@@ -126,6 +127,54 @@ def test_sparse_readouts_and_logic_programs(discrete):
     if discrete:
         c += "\ndiscrete_time = 1;\n"
     _run([dict(bnd=b, cfg=c, n=300, seed=4)], grid=2, block=64, wsum_reg=1)
+
+
+def _check_errors(g, r):
+    assert g["err_miss"] == 0
+    assert g["FQ"] == r.sq_exponent and g["FQT"] == r.th_sq_exponent
+    assert np.array_equal(g["th2"], r.th_sq), "TH sums of squares differ"
+    if r.readouts > 20:
+        taken = g["cells"] != np.uint64(0xFFFFFFFFFFFFFFFF)
+        keys, sq = g["cells"][taken], g["cellv2"][taken]
+        order = np.argsort(keys, kind="stable")
+        keys, sq = keys[order], sq[order]
+        inner = (keys >> np.uint64(r.readouts)) < np.uint64(r.windows)
+        assert np.array_equal(sq[inner], r.cell_squares), "cell sums of squares differ"
+    else:
+        assert np.array_equal(g["prob2"], r.prob_sq.ravel()), "window sums of squares differ"
+
+
+@pytest.mark.parametrize("case", ["dense_act_hbm", "discrete", "sparse", "sweep"])
+def test_error_column_sums_bit_exact(case):
+    """MBG_ERRORS: each lane's window list of (state, time), merged on revisits and flushed with
+    its squares when the window ends, gives the oracle's sums of squares bit for bit."""
+    if case == "dense_act_hbm":
+        v = _net(1000, count=48, time_tick=2.0)
+        vs, over = [v], dict(block=128, tables_lds=1, wsum_reg=1, state_hbm=2)
+    elif case == "discrete":
+        b, c = model_text("cellcycle")
+        vs, over = [dict(bnd=b, cfg=c, n=300, seed=7)], dict(block=64, wsum_reg=1, tables_lds=1)
+    elif case == "sparse":
+        net = synth.random_network(60, 2, 4, seed=9, rates="loguniform", istate="uniform", readouts=24,
+                                   max_time=6.0, time_tick=1.5, sample_count=150)
+        vs, over = [dict(bnd=net.bnd(), cfg=net.cfg(), n=150, seed=3)], dict(block=64, wsum_reg=1)
+    else:
+        vs = [_net(n, seed=s, count=c, time_tick=1.0) for n, s, c in [(300, 4, 50), (420, 5, 30)]]
+        for k, v in enumerate(vs):
+            v.update(seed=20 + k, vid=k)
+        over = dict(nw=8, block=128, multi=1, wsum_reg=1, state_hbm=2)
+    sh = emu.shape_for(vs, errors=1, **over)
+    exe = emu.build(sh)
+    gs = emu.run(exe, vs, grid=2, err_cap=32)
+    for v, g in zip(vs, gs):
+        assert g["err_miss"] == 0
+        r = _oracle(v, trace=False)
+        _check(g, r, False)
+        _check_errors(g, r)
+    # a list of one entry overflows on a window with two states; the miss is counted (the
+    # runtime then repeats the run with a longer list)
+    if case == "dense_act_hbm":
+        assert emu.run(exe, vs, grid=2, err_cap=1)[0]["err_miss"] > 0
 
 
 def test_generic_rates_and_batch_resume():

@@ -209,6 +209,60 @@ def test_variant_overrides_match_exact_ctmc_of_the_rewritten_network(k):
     assert np.array_equal(r2.prob, r.prob) and np.array_equal(r2.final, r.final) and r2.flips == r.flips
 
 
+@pytest.mark.parametrize("discrete", [False, True])
+def test_error_column_sums_equal_brute_force_per_trajectory(discrete):
+    """The sums of squares behind ErrorProba / ErrorTH equal sum_i f_i^2 computed in NumPy from
+    every trajectory's own tables: each trajectory run alone (count 1: its window table is its
+    time per (window, state), revisits within a window included) gives f_i[w][s] and its
+    window-average TH; a wrong revisit merge, a dropped square or a wrong scale fails this."""
+    net = synth.random_network(6, 1, 3, seed=8, rates="loguniform", istate="uniform", readouts=4,
+                               max_time=6.0, time_tick=2.0, sample_count=60)
+    cfg = net.cfg() + ("discrete_time = 1;\n" if discrete else "")
+    m = O.OracleModel(net.bnd(), cfg)
+    n = 60
+    r = m.run(count=n, n_trajectories=n, seed=3, threads=4)
+    tick = float(np.float32(net.time_tick))
+    f2 = np.zeros(r.prob.shape)
+    t2 = np.zeros(r.windows)
+    revisits = 0
+    for i in range(n):
+        one = m.run(count=1, first_id=i, n_trajectories=1, seed=3, threads=1)
+        f = one.prob.astype(np.float64) / 2.0 ** one.prob_exponent
+        t = one.th.astype(np.float64) / 2.0 ** one.th_exponent
+        if not discrete:
+            f /= tick
+            t /= tick
+        f2 += f * f
+        t2 += t * t
+        revisits += int(np.sum((f > 0) & (f < 1)))
+    got = r.prob_sq.astype(np.float64) / 2.0 ** r.sq_exponent
+    gth = r.th_sq.astype(np.float64) / 2.0 ** r.th_sq_exponent
+    assert np.allclose(got, f2, rtol=1e-9, atol=n * 2.0 ** -40)
+    assert np.allclose(gth, t2, rtol=1e-9, atol=n * 2.0 ** -40)
+    if not discrete:
+        assert revisits > 0  # windows shared by several states: partial fractions occur
+    # the same sums from the cell list (window-major, states ascending)
+    w, st, _ = r.cells
+    assert np.array_equal(r.cell_squares, r.prob_sq[w, st])
+
+
+def test_error_columns_closed_form_for_frozen_states():
+    """A network whose nodes never switch: each trajectory keeps its initial projected state, so
+    every fraction is 0 or 1, E[f^2] = P and the standard error is the binomial one,
+    sqrt(P (1 - P) / (n - 1)); every TH is 0."""
+    bnd = ("node A { logic = A; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }\n"
+           "node B { logic = B; rate_up = @logic ? 2 : 0; rate_down = @logic ? 0 : 2; }\n")
+    cfg = "A.istate = 0.3; B.istate = 0.6; max_time = 3; time_tick = 1; sample_count = 5000;"
+    r = O.OracleModel(bnd, cfg).run(count=5000, threads=4)
+    n = 5000
+    P = r.prob.astype(np.float64) / 2.0 ** r.prob_exponent / n
+    E2 = r.prob_sq.astype(np.float64) / 2.0 ** r.sq_exponent / n
+    assert np.allclose(E2, P, rtol=0, atol=1e-12)
+    se = np.sqrt((E2 - P * P).clip(0) / (n - 1))
+    assert np.allclose(se, np.sqrt(P * (1 - P) / (n - 1)), rtol=0, atol=1e-9)
+    assert not np.any(r.th_sq)
+
+
 def _discrete_exact(net_rates, p0, proj, K, nsteps):
     """Jump chain with the reference's discrete-time window rule (a fixed point is counted only in
     the window where it is reached, src/jit_kernels/window_average_small.cu:101-136)."""

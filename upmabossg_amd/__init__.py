@@ -58,6 +58,7 @@ class _RunParams(C.Structure):
         ("block", C.c_int),
         ("grid", C.c_int),
         ("compat_batch_resume", C.c_int),
+        ("error_columns", C.c_int),
     ]
 
 
@@ -69,7 +70,7 @@ class _RunInfo(C.Structure):
         ("fixed_point_trajectories", C.c_uint64), ("kernel_ms", C.c_double),
         ("time_tick", C.c_float), ("max_time", C.c_float),
         ("block", C.c_int), ("tables_in_lds", C.c_int), ("lds_bytes", C.c_int), ("blocks_per_cu", C.c_int),
-        ("state_hbm", C.c_int),
+        ("state_hbm", C.c_int), ("error_columns", C.c_int), ("sq_exponent", C.c_int), ("th_sq_exponent", C.c_int),
     ]
 
 
@@ -127,12 +128,17 @@ def lib():
         "mbg_get_probtraj_raw": (C.c_int, [vp, u64p, u64p]),
         "mbg_get_probtraj": (C.c_int, [vp, f64p, f64p]),
         "mbg_get_final_states": (C.c_int, [vp, u64p]),
+        "mbg_get_probtraj_errors": (C.c_int, [vp, f64p, f64p]),
+        "mbg_get_probtraj_squares_raw": (C.c_int, [vp, u64p, u64p]),
+        "mbg_get_probtraj_cell_squares": (C.c_int, [vp, u64p, sz, C.POINTER(sz)]),
         "mbg_get_probtraj_cells": (C.c_int, [vp, C.POINTER(C.c_uint32), u64p, u64p, sz, C.POINTER(sz)]),
         "mbg_get_final_cells": (C.c_int, [vp, u64p, u64p, sz, C.POINTER(sz)]),
         "mbg_get_fixed_points": (C.c_int, [vp, u64p, u64p, sz, C.POINTER(sz)]),
         "mbg_get_summaries": (C.c_int, [vp, u64p, sz]),
         "mbg_get_marginals": (C.c_int, [vp, f64p]),
         "mbg_run_variants": (C.c_int, [vp, C.POINTER(_Variant), C.c_int]),
+        "mbg_run_variants_ex": (C.c_int, [vp, C.POINTER(_Variant), C.c_int, C.c_int]),
+        "mbg_get_variant_probtraj_squares_raw": (C.c_int, [vp, C.c_int, u64p, u64p]),
         "mbg_get_sweep_info": (C.c_int, [vp, C.POINTER(_SweepInfo)]),
         "mbg_get_variant_info": (C.c_int, [vp, C.c_int, C.POINTER(_RunInfo)]),
         "mbg_get_variant_probtraj_raw": (C.c_int, [vp, C.c_int, u64p, u64p]),
@@ -318,6 +324,15 @@ class RunResult:
     # with states ascending, and the final states (state, count), states ascending
     cells: tuple | None = None
     final_cells: tuple | None = None
+    # error columns (runs with error_columns=True): sums of squared per-trajectory window totals
+    # and the standard errors they give (ErrorProba, ErrorTH)
+    prob_sq: np.ndarray | None = None      # (W, 2^K) uint64, units 2^-sq_exponent
+    th_sq: np.ndarray | None = None        # (W,) uint64, units 2^-th_sq_exponent
+    cell_squares: np.ndarray | None = None  # above 20 readouts, aligned with cells
+    prob_error: np.ndarray | None = None   # (W, 2^K) float64
+    th_error: np.ndarray | None = None     # (W,) float64
+    sq_exponent: int = 0
+    th_sq_exponent: int = 0
 
 
 class Context:
@@ -352,11 +367,11 @@ class Context:
         self._check(lib().mbg_compile(self._h, model.handle, int(trace)))
 
     def run(self, model: Model, n_trajectories=0, count=0, first_id=0, seed=None, variant_id=0, trace=False,
-            block=0, grid=0, fetch=True, compat_batch_resume=False) -> RunResult | None:
+            block=0, grid=0, fetch=True, compat_batch_resume=False, error_columns=False) -> RunResult | None:
         if count and not n_trajectories:
             n_trajectories = count  # a partial run is normalised by its own count unless told otherwise
         p = _RunParams(n_trajectories, first_id, count, 0 if seed is None else seed, int(seed is None),
-                       variant_id, int(trace), block, grid, int(compat_batch_resume))
+                       variant_id, int(trace), block, grid, int(compat_batch_resume), int(error_columns))
         self._check(lib().mbg_run(self._h, model.handle, C.byref(p)))
         return self.result(trace) if fetch else None
 
@@ -421,9 +436,29 @@ class Context:
         if with_summaries:
             summ = np.zeros((i.simulated, 4), np.uint64)
             self._check(L.mbg_get_summaries(self._h, _u64p(summ), i.simulated))
-        return RunResult(W, i.readouts, i.prob_exponent, i.th_exponent, i.rate_exponent, bool(i.discrete),
-                         i.n_trajectories, i.simulated, i.transitions, i.fixed_point_trajectories, i.kernel_ms,
-                         i.time_tick, i.max_time, prob_raw, th_raw, prob, th, fin, fs, fc, summ, cells, final_cells)
+        res = RunResult(W, i.readouts, i.prob_exponent, i.th_exponent, i.rate_exponent, bool(i.discrete),
+                        i.n_trajectories, i.simulated, i.transitions, i.fixed_point_trajectories, i.kernel_ms,
+                        i.time_tick, i.max_time, prob_raw, th_raw, prob, th, fin, fs, fc, summ, cells, final_cells)
+        if i.error_columns:
+            res.sq_exponent, res.th_sq_exponent = i.sq_exponent, i.th_sq_exponent
+            res.th_sq = np.zeros(W, np.uint64)
+            res.th_error = np.zeros(W, np.float64)
+            if i.readouts > 20:
+                self._check(L.mbg_get_probtraj_squares_raw(self._h, None, _u64p(res.th_sq)))
+                self._check(L.mbg_get_probtraj_errors(self._h, None, res.th_error.ctypes.data_as(C.POINTER(C.c_double))))
+                n = C.c_size_t()
+                self._check(L.mbg_get_probtraj_cell_squares(self._h, None, 0, C.byref(n)))
+                res.cell_squares = np.zeros(n.value, np.uint64)
+                self._check(L.mbg_get_probtraj_cell_squares(self._h, _u64p(res.cell_squares), n.value, C.byref(n)))
+            else:
+                S = 1 << i.readouts
+                res.prob_sq = np.zeros(W * S, np.uint64)
+                res.prob_error = np.zeros(W * S, np.float64)
+                self._check(L.mbg_get_probtraj_squares_raw(self._h, _u64p(res.prob_sq), _u64p(res.th_sq)))
+                self._check(L.mbg_get_probtraj_errors(self._h, res.prob_error.ctypes.data_as(C.POINTER(C.c_double)),
+                                                      res.th_error.ctypes.data_as(C.POINTER(C.c_double))))
+                res.prob_sq, res.prob_error = res.prob_sq.reshape(W, S), res.prob_error.reshape(W, S)
+        return res
 
     def probtraj_raw(self, out=None) -> np.ndarray:
         """The last run's window table (W * 2^K exact cells), copied to host memory."""
@@ -442,17 +477,27 @@ class Context:
         return out.reshape(i.windows, i.readouts)
 
     # ---- variant sweeps ----
-    def run_variants(self, variants):
+    def run_variants(self, variants, error_columns=False):
         """variants: iterable of (Model, n_trajectories, seed or None, variant_id).  All run in one
-        HIP graph launch; returns the sweep info (per-variant results through the getters below)."""
+        HIP graph launch; returns the sweep info (per-variant results through the getters below).
+        error_columns: also accumulate the error columns' sums of squares (variant_squares)."""
         vs = list(variants)
         arr = (_Variant * len(vs))()
         self._sweep_models = [v[0] for v in vs]  # keep the models alive with the device results
         for k, (model, n, seed, vid) in enumerate(vs):
             arr[k] = _Variant(model.handle.value if hasattr(model.handle, "value") else model.handle, n,
                               0 if seed is None else seed, int(seed is None), vid)
-        self._check(lib().mbg_run_variants(self._h, arr, len(vs)))
+        self._check(lib().mbg_run_variants_ex(self._h, arr, len(vs), 1 if error_columns else 0))
         return self.sweep_info()
+
+    def variant_squares(self, v):
+        """(prob_sq (W, 2^K), th_sq (W,)) of variant v of a sweep run with error_columns."""
+        i = self.variant_info(v)
+        W, S = i.windows, 1 << i.readouts
+        prob = np.zeros(W * S, np.uint64)
+        th = np.zeros(W, np.uint64)
+        self._check(lib().mbg_get_variant_probtraj_squares_raw(self._h, v, _u64p(prob), _u64p(th)))
+        return prob.reshape(W, S), th
 
     def sweep_info(self) -> _SweepInfo:
         i = _SweepInfo()

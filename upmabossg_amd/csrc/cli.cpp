@@ -60,6 +60,13 @@ bool load_variants(const mbg_model* base, const std::string& path, std::vector<m
     return true;
 }
 
+// ErrorTH, HD=0 and ErrorProba are filled in (MaBoSS's standard errors of the window averages);
+// MBG_ERROR_COLUMNS=0 writes zeros there, as the reference does.
+int error_columns() {
+    const char* ec = std::getenv("MBG_ERROR_COLUMNS");
+    return ec && std::atoi(ec) == 0 ? 0 : 1;
+}
+
 // More than 20 readouts: sweeps keep dense tables only, so each variant is a plain run (same
 // trajectory count, seed and draws as in a sweep).
 int run_variants_one_by_one(mbg_ctx* ctx, const std::vector<mbg_variant>& vs, const std::vector<std::string>& names,
@@ -70,6 +77,7 @@ int run_variants_one_by_one(mbg_ctx* ctx, const std::vector<mbg_variant>& vs, co
         p.seed = vs[i].seed;
         p.use_model_seed = vs[i].use_model_seed;
         p.variant_id = vs[i].variant_id;
+        p.error_columns = error_columns();
         if (mbg_run(ctx, vs[i].model, &p) != MBG_OK) {
             std::cerr << mbg_last_error(ctx) << std::endl;
             return 1;
@@ -98,7 +106,7 @@ int run_variants(mbg_ctx* ctx, const mbg_model* base, const std::vector<mbg_vari
     const int group = std::max(1, std::min(256, (int)(budget / table_bytes(base))));
     for (int g0 = 0; g0 < count; g0 += group) {
         const int g = std::min(group, count - g0);
-        if (mbg_run_variants(ctx, vs.data() + g0, g) != MBG_OK) {
+        if (mbg_run_variants_ex(ctx, vs.data() + g0, g, error_columns() ? MBG_SWEEP_ERROR_COLUMNS : 0) != MBG_OK) {
             std::cerr << mbg_last_error(ctx) << std::endl;
             return 1;
         }
@@ -179,6 +187,7 @@ int main(int argc, char** argv) {
         // time for the same state after every 99th step) instead of uninterrupted Gillespie
         const char* br = std::getenv("MBG_BATCH_RESUME");
         p.compat_batch_resume = br && std::atoi(br) != 0 ? 1 : 0;
+        p.error_columns = error_columns();
         if (mbg_run(ctx, model, &p) != MBG_OK) {
             std::cerr << mbg_last_error(ctx) << std::endl;
             rc = 1;

@@ -23,6 +23,9 @@
 //   MBG_TH_LDS      1: partial sums of the window entropies in LDS (per block in single runs,
 //                   per wave and variant in sweeps)
 //   MBG_CELLS_LDS   1: also of the whole window table and final states (small tables)
+//   MBG_ERRORS      1: also the sums of squares behind the ErrorProba / ErrorTH columns (each
+//                   trajectory's time per projected state and TH x time over each window,
+//                   squared); each lane keeps its current window's (state, time) list in HBM
 //   MBG_STATE_HBM   1: the lanes' x, act and word sums in an HBM buffer (lane-interleaved over
 //                   the whole grid) instead of LDS, for models whose per-lane state would leave
 //                   a CU with one or two waves (single runs only); 2: only act in that buffer,
@@ -110,7 +113,10 @@ struct mbg_node {
 #ifndef MBG_CELLS_LDS
 #define MBG_CELLS_LDS 0
 #endif
-#if !MBG_TH_LDS || MBG_SPARSE || MBG_MULTI  // the cell partials follow the entropy partials;
+#ifndef MBG_ERRORS
+#define MBG_ERRORS 0
+#endif
+#if !MBG_TH_LDS || MBG_SPARSE || MBG_MULTI || MBG_ERRORS  // the cell partials follow the entropy partials;
                                            // dense tables of single runs only
 #undef MBG_CELLS_LDS
 #define MBG_CELLS_LDS 0
@@ -224,6 +230,20 @@ struct mbg_desc {
     u64* lane_state;
     u32 lane_stride;
     u32 pad2_;
+    // MBG_ERRORS: sums over trajectories of squared per-trajectory window totals, prob2 [W][2^K]
+    // (cell_vals2 beside cell_vals when sparse) and th2 [W], in units 2^-FQ (2^-FQT) of the
+    // squared window fraction (window-average TH), a total v scaling to it by sq_scale
+    // (sq_scale_th); each lane's list of the current window's (state, time) entries, entry j of
+    // grid thread t at j * err_stride + t, at most err_cap entries (err_miss counts overflows)
+    u64* prob2;
+    u64* th2;
+    u64* cell_vals2;
+    u64* err_s;
+    u64* err_v;
+    u64* err_miss;
+    double sq_scale, sq_scale_th;
+    int FQ, FQT;
+    u32 err_cap, err_stride;
 };
 
 #if MBG_COMPACT
@@ -703,7 +723,7 @@ struct Lane {
 // written once (compare-and-swap on empty) and never moves, so no lane waits on another; a
 // key that finds no slot is counted in cell_miss and the host repeats the run with a larger
 // table (the run is deterministic, so the repeat gives the same cells).
-__device__ __forceinline__ void sparse_add(const mbg_desc& D, u64 key, u64 v) {
+__device__ __forceinline__ void sparse_add(const mbg_desc& D, u64 key, u64 v, u64 q = 0) {
     u64 h = key;
     h ^= h >> 33;
     h *= 0xff51afd7ed558ccdull;
@@ -719,6 +739,11 @@ __device__ __forceinline__ void sparse_add(const mbg_desc& D, u64 key, u64 v) {
         }
         if (cur == key) {
             atomicAdd(D.cell_vals + slot, v);
+#if MBG_ERRORS
+            if (q) atomicAdd(D.cell_vals2 + slot, q);
+#else
+            (void)q;
+#endif
             return;
         }
     }
@@ -757,6 +782,78 @@ __device__ __forceinline__ void flush_th(const mbg_desc& D, u64* thl, int w, u64
 #endif
 }
 
+#if MBG_ERRORS
+// Square of a trajectory's window total v in the units of its sum of squares (the oracle's
+// sq_units): rint((v * scale)^2 * 2^F).
+__device__ __forceinline__ u64 sq_units(u64 v, double scale, int F) {
+    const double x = (double)v * scale;
+    return (u64)rint(ldexp(x * x, F));
+}
+
+// A visit of projected state s lasting v ends inside the current window: its time joins the
+// lane's list of the window's states (a trajectory may return to a state within a window, and
+// the error columns square the window's total per state).  ne: entries in the list.
+__device__ __forceinline__ void err_visit(const mbg_desc& D, u32 t, u32& ne, sidx s, u64 v) {
+    if (!v) return;
+    for (u32 j = 0; j < ne; j++) {
+        const u64 at = (u64)j * D.err_stride + t;
+        if (D.err_s[at] == (u64)s) {
+            D.err_v[at] += v;
+            return;
+        }
+    }
+    if (ne < D.err_cap) {
+        const u64 at = (u64)ne * D.err_stride + t;
+        D.err_s[at] = (u64)s;
+        D.err_v[at] = v;
+        ne++;
+    } else {
+        atomicAdd(D.err_miss, 1ull);  // the host repeats the run with a longer list
+    }
+}
+
+// The window w ends: every listed state's total and its square go to the tables, and the
+// window's TH total th and its square (squares of TH: th2, or after the entropies in LDS).
+__device__ __forceinline__ void err_window(const mbg_desc& D, u64* thl, u32 t, u32& ne, int w, u64 th) {
+    for (u32 j = 0; j < ne; j++) {
+        const u64 at = (u64)j * D.err_stride + t;
+        const sidx s = (sidx)D.err_s[at];
+        const u64 v = D.err_v[at];
+        const u64 q = sq_units(v, D.sq_scale, D.FQ);
+#if MBG_SPARSE
+        sparse_add(D, ((u64)w << D.K) | s, v, q);
+#else
+        atomicAdd(D.prob + ((u64)w << D.K) + s, v);
+        if (q) atomicAdd(D.prob2 + ((u64)w << D.K) + s, q);
+#endif
+    }
+    ne = 0;
+    if (th) {
+        flush_th(D, thl, w, th);
+        const u64 q = sq_units(th, D.sq_scale_th, D.FQT);
+#if MBG_TH_LDS
+        if (q) atomicAdd(thl + D.W + w, q);
+#else
+        if (q) atomicAdd(D.th2 + w, q);
+#endif
+    }
+}
+// the current visit (acc_s, acc_p) ends; the current window cur_w ends
+#define END_VISIT() err_visit(D, etid, ne, acc_s, acc_p)
+#define END_WINDOW()                                   \
+    do {                                               \
+        err_visit(D, etid, ne, acc_s, acc_p);          \
+        err_window(D, thl, etid, ne, cur_w, acc_th);   \
+    } while (0)
+#else
+#define END_VISIT() flush_cell(D, thl, cur_w, acc_s, acc_p)
+#define END_WINDOW()                                        \
+    do {                                                    \
+        flush_cell(D, thl, cur_w, acc_s, acc_p);            \
+        if (acc_th) flush_th(D, thl, cur_w, acc_th);        \
+    } while (0)
+#endif
+
 __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* tb, u64* lane, u64* thl) {
     // index of this lane's state: its thread in the block (LDS) or in the grid (HBM)
     const u32 tid = MBG_STATE_HBM == 1 ? blockIdx.x * MBG_BLOCK + threadIdx.x : threadIdx.x;
@@ -764,6 +861,10 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
     const u32 atid = blockIdx.x * MBG_BLOCK + threadIdx.x;  // this lane's column of act in HBM
 #endif
     const u32 lid = threadIdx.x & 63u;
+#if MBG_ERRORS
+    const u32 etid = blockIdx.x * MBG_BLOCK + threadIdx.x;  // this lane's column of the error lists
+    u32 ne = 0;
+#endif
     Lane L{D, tb, lane, tid};
     const u32 key0 = (u32)D.seed, key1 = (u32)(D.seed >> 32);
 
@@ -848,6 +949,9 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     acc_s = s;
                     dirty = true;
                     flips = 0;
+#if MBG_ERRORS
+                    ne = 0;
+#endif
 #if MBG_TRACE
                     hash = 0xcbf29ce484222325ull;
 #endif
@@ -992,16 +1096,13 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             int w = (int)floor(t * D.inv_tick + 0.5);
             if (w > D.W - 1) w = D.W - 1;
             if (w != cur_w) {
-                if (cur_w >= 0) {
-                    flush_cell(D, thl, cur_w, acc_s, acc_p);
-                    if (acc_th) flush_th(D, thl, cur_w, acc_th);
-                }
+                if (cur_w >= 0) END_WINDOW();
                 cur_w = w;
                 acc_p = 0;
                 acc_th = 0;
                 acc_s = s;
             } else if (s != acc_s) {
-                flush_cell(D, thl, cur_w, acc_s, acc_p);
+                END_VISIT();
                 acc_p = 0;
                 acc_s = s;
             }
@@ -1018,16 +1119,13 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 const double sl = e - b;
                 const int ww = w > D.W - 1 ? D.W - 1 : w;
                 if (ww != cur_w) {
-                    if (cur_w >= 0) {
-                        flush_cell(D, thl, cur_w, acc_s, acc_p);
-                        if (acc_th) flush_th(D, thl, cur_w, acc_th);
-                    }
+                    if (cur_w >= 0) END_WINDOW();
                     cur_w = ww;
                     acc_p = 0;
                     acc_th = 0;
                     acc_s = s;
                 } else if (s != acc_s) {
-                    flush_cell(D, thl, cur_w, acc_s, acc_p);
+                    END_VISIT();
                     acc_p = 0;
                     acc_s = s;
                 }
@@ -1042,10 +1140,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 
         if (t >= D.maxt) {
             // ---- trajectory end: flush, final state, fixed point, summary ----
-            if (cur_w >= 0) {
-                flush_cell(D, thl, cur_w, acc_s, acc_p);
-                if (acc_th) flush_th(D, thl, cur_w, acc_th);
-            }
+            if (cur_w >= 0) END_WINDOW();
 #if MBG_SPARSE
             sparse_add(D, ((u64)D.W << D.K) | s, 1ull);
 #elif MBG_CELLS_LDS
@@ -1257,6 +1352,8 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
 #if MBG_CELLS_LDS
     const u32 S = 1u << D.K;
     const u32 nparts = (u32)D.W + ((u32)D.W + 1u) * S;
+#elif MBG_ERRORS
+    const u32 nparts = 2u * (u32)D.W;  // th[W], then the squares th2[W]
 #else
     const u32 nparts = (u32)D.W;
 #endif
@@ -1272,6 +1369,9 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
         if (i < (u32)D.W) atomicAdd(D.th + i, v);
         else if (c < (u32)D.W * S) atomicAdd(D.prob + c, v);
         else atomicAdd(D.fin + (c - (u32)D.W * S), v);
+#elif MBG_ERRORS
+        if (i < (u32)D.W) atomicAdd(D.th + i, v);
+        else atomicAdd(D.th2 + (i - (u32)D.W), v);
 #else
         atomicAdd(D.th + i, v);
 #endif
@@ -1294,9 +1394,11 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
     const u32 wave = blockIdx.x * (MBG_BLOCK / 64) + (threadIdx.x >> 6);
     const int start = (int)(((u64)wave * (u64)nv) / waves_total);
 #if MBG_TH_LDS
+    // the slice holds the variant's W entropies, then (MBG_ERRORS) their W squares
     const u32 lid = threadIdx.x & 63u;
+    const u32 th_cells = (MBG_ERRORS ? 2u : 1u);
     u64* thw = smem + (MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK +
-               (threadIdx.x >> 6) * (u32)wmax;
+               (threadIdx.x >> 6) * th_cells * (u32)wmax;
 #else
     (void)wmax;
     u64* thw = nullptr;
@@ -1307,7 +1409,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
         v = __builtin_amdgcn_readfirstlane(v);
         const mbg_desc& D = descs[v];
 #if MBG_TH_LDS
-        for (u32 c = lid; c < (u32)D.W; c += 64u) thw[c] = 0;
+        for (u32 c = lid; c < th_cells * (u32)D.W; c += 64u) thw[c] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
 #endif
@@ -1315,9 +1417,11 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
 #if MBG_TH_LDS
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();  // every lane of the wave has left the variant's loop
-        for (u32 c = lid; c < (u32)D.W; c += 64u) {
+        for (u32 c = lid; c < th_cells * (u32)D.W; c += 64u) {
             const u64 x = thw[c];
-            if (x) atomicAdd(D.th + c, x);
+            if (!x) continue;
+            if (c < (u32)D.W) atomicAdd(D.th + c, x);
+            else atomicAdd(D.th2 + (c - (u32)D.W), x);
         }
 #endif
     }
@@ -1453,8 +1557,10 @@ extern "C" __global__ void __launch_bounds__(256) mbg_cells_count(const u64* key
     if (threadIdx.x == 0 && total) atomicAdd(out_n, (u64)total);
 }
 
-extern "C" __global__ void __launch_bounds__(256) mbg_cells_compact(const u64* keys, const u64* vals, u64 cap,
-                                                                    u64* out_keys, u64* out_vals, u64* out_n,
+// vals2 / out_vals2 (may be null): the sums of squares beside the values (error columns).
+extern "C" __global__ void __launch_bounds__(256) mbg_cells_compact(const u64* keys, const u64* vals,
+                                                                    const u64* vals2, u64 cap, u64* out_keys,
+                                                                    u64* out_vals, u64* out_vals2, u64* out_n,
                                                                     u64 out_cap) {
     __shared__ u32 lds[4];
     __shared__ u64 base;
@@ -1468,5 +1574,6 @@ extern "C" __global__ void __launch_bounds__(256) mbg_cells_compact(const u64* k
     if (has && pos < out_cap) {  // out_n above out_cap is reported to the host as an error
         out_keys[pos] = keys[i];
         out_vals[pos] = vals[i];
+        if (vals2) out_vals2[pos] = vals2[i];
     }
 }

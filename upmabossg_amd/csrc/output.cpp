@@ -25,6 +25,23 @@ double window_th(const Results& r, size_t w) {
     return v;
 }
 
+double standard_error(const mbg_run_info& I, double mean, uint64_t raw_sq, int exponent) {
+    const double n = (double)I.n_trajectories;
+    if (n < 2.0) return 0.0;
+    const double var = std::ldexp((double)raw_sq, -exponent) / n - mean * mean;
+    return var > 0.0 ? std::sqrt(var / (n - 1.0)) : 0.0;
+}
+
+double window_prob_error(const Results& r, size_t cell) {
+    if (!r.prob2) return 0.0;
+    return standard_error(*r.info, window_prob(r, cell), (*r.prob2)[cell], r.info->sq_exponent);
+}
+
+double window_th_error(const Results& r, size_t w) {
+    if (!r.th2) return 0.0;
+    return standard_error(*r.info, window_th(r, w), (*r.th2)[w], r.info->th_sq_exponent);
+}
+
 std::string state_string(const CompiledModel& cm, const uint64_t* words) {
     std::string s;
     for (int i = 0; i < cm.N; i++)
@@ -48,30 +65,44 @@ std::string projected_string(const CompiledModel& cm, uint64_t st) {
 namespace {
 
 struct WindowRow {
-    float time, th, entropy;
-    std::vector<std::pair<uint64_t, float>> probs;  // non-zero states ascending
+    float time, th, th_err, entropy, hd0;
+    struct Cell {
+        uint64_t state;
+        float p, err;
+    };
+    std::vector<Cell> probs;  // non-zero states ascending
 };
 
+// One window of the probtraj output.  HD=0 is the probability that the state is at Hamming
+// distance 0 from the reference state; the language has no reference state, so every state is
+// at distance 0 and the column is the window's total probability.
 WindowRow window_row(const Results& r, size_t w) {
+    const mbg_run_info& I = *r.info;
     WindowRow row;
-    row.time = (float)w * r.info->time_tick;
-    row.th = (float)window_th(r, w);
+    row.time = (float)w * I.time_tick;
+    const double th = window_th(r, w);
+    row.th = (float)th;
+    row.th_err = r.th2 ? (float)standard_error(I, th, (*r.th2)[w], I.th_sq_exponent) : 0.f;
     row.entropy = 0.f;
-    auto add = [&](uint64_t s, uint64_t raw) {
-        const float p = (float)cell_prob(*r.info, raw);
+    double total = 0.0;
+    auto add = [&](uint64_t s, uint64_t raw, const uint64_t* raw_sq) {
+        const double pd = cell_prob(I, raw);
+        const float p = (float)pd;
         if (p == 0.f) return;
         row.entropy += -std::log2(p) * p;
-        row.probs.emplace_back(s, p);
+        total += pd;
+        row.probs.push_back({s, p, raw_sq ? (float)standard_error(I, pd, *raw_sq, I.sq_exponent) : 0.f});
     };
     if (r.cell_w) {
         const auto lo = std::lower_bound(r.cell_w->begin(), r.cell_w->end(), (uint32_t)w);
         const auto hi = std::upper_bound(lo, r.cell_w->end(), (uint32_t)w);
         for (size_t i = (size_t)(lo - r.cell_w->begin()); i < (size_t)(hi - r.cell_w->begin()); i++)
-            add((*r.cell_s)[i], (*r.cell_v)[i]);
+            add((*r.cell_s)[i], (*r.cell_v)[i], r.cell_v2 ? &(*r.cell_v2)[i] : nullptr);
     } else {
-        const uint64_t S = 1ull << r.info->readouts;
-        for (uint64_t s = 0; s < S; s++) add(s, (*r.prob)[w * S + s]);
+        const uint64_t S = 1ull << I.readouts;
+        for (uint64_t s = 0; s < S; s++) add(s, (*r.prob)[w * S + s], r.prob2 ? &(*r.prob2)[w * S + s] : nullptr);
     }
+    row.hd0 = r.th2 ? (float)total : 0.f;
     return row;
 }
 
@@ -90,9 +121,11 @@ bool write_probtraj_csv(const Results& r, const CompiledModel& cm, const std::st
     ofs << "Time\tTH\tErrorTH\tH\tHD=0";
     for (size_t i = 0; i < max_states; i++) ofs << "\tState\tProba\tErrorProba";
     ofs << std::endl;
+    // ErrorTH, HD=0 and ErrorProba are 0 when the run did not accumulate the error columns, as
+    // the reference always writes them (window_average_small.cpp:235,245)
     for (const WindowRow& row : rows) {
-        ofs << row.time << "\t" << row.th << "\t" << 0.f << "\t" << row.entropy << "\t" << 0.f;
-        for (const auto& sp : row.probs) ofs << "\t" << projected_string(cm, sp.first) << "\t" << sp.second << "\t" << 0.f;
+        ofs << row.time << "\t" << row.th << "\t" << row.th_err << "\t" << row.entropy << "\t" << row.hd0;
+        for (const auto& c : row.probs) ofs << "\t" << projected_string(cm, c.state) << "\t" << c.p << "\t" << c.err;
         ofs << std::endl;
     }
     return (bool)ofs;
@@ -135,7 +168,7 @@ void write_text(const Results& r, const CompiledModel& cm, std::ostream& os) {
         os << "window (" << (float)w * tick << ", " << (float)(w + 1) * tick << "]" << std::endl;
         os << "entropy: " << row.entropy << std::endl;
         os << "transition entropy: " << row.th << std::endl;
-        for (const auto& sp : row.probs) os << sp.second << " " << projected_string(cm, sp.first) << std::endl;
+        for (const auto& c : row.probs) os << c.p << " " << projected_string(cm, c.state) << std::endl;
     }
 }
 

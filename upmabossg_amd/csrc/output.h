@@ -39,6 +39,11 @@ struct Results {
     const std::vector<uint64_t>* cell_v = nullptr;
     const std::vector<uint64_t>* fin_s = nullptr;
     const std::vector<uint64_t>* fin_v = nullptr;
+    // Error columns (null when the run did not accumulate them: the columns print 0): sums of
+    // squared per-trajectory window totals, beside prob (or cell_v) and th.
+    const std::vector<uint64_t>* prob2 = nullptr;
+    const std::vector<uint64_t>* cell_v2 = nullptr;
+    const std::vector<uint64_t>* th2 = nullptr;
 };
 
 // Normalised probability of a raw cell value.
@@ -48,6 +53,13 @@ double cell_prob(const mbg_run_info& I, uint64_t v);
 // (src/statistics/window_average_small.cpp:126-140, :222-223).
 double window_prob(const Results& r, size_t cell);
 double window_th(const Results& r, size_t w);
+
+// Standard error of the mean over the n trajectories of a per-trajectory window quantity with
+// mean `mean` and sum of squares raw_sq (units 2^-exponent): sqrt((E[x^2] - mean^2) / (n - 1)),
+// 0 when n < 2 or the difference rounds below 0.  The ErrorProba and ErrorTH columns.
+double standard_error(const mbg_run_info& I, double mean, uint64_t raw_sq, int exponent);
+double window_prob_error(const Results& r, size_t cell);
+double window_th_error(const Results& r, size_t w);
 
 // " -- "-joined names of the ON nodes, or "<nil>" (src/state.cpp:29-48).
 std::string state_string(const CompiledModel& cm, const uint64_t* words);

@@ -97,6 +97,13 @@ RunScales run_scales(const CompiledModel& cm, uint64_t n) {
     }
     const float ratio = cm.max_time / cm.time_tick;
     r.nsteps = ratio >= 2.f ? (uint64_t)std::floor(ratio) : 1;
+    // a window fraction is at most 1 and a window-average TH below 64 (at most 6 bits of
+    // entropy over 63 readouts): n squares stay below 2^62
+    r.FQ = 62 - ceil_log2_u64(n);
+    r.FQT = 56 - ceil_log2_u64(n);
+    const double tick = (double)cm.time_tick;
+    r.sq_scale = cm.discrete ? 1.0 : std::ldexp(1.0, -r.FP) / tick;
+    r.sq_scale_th = cm.discrete ? std::ldexp(1.0, -r.FT) : std::ldexp(1.0, -r.FT) / tick;
     return r;
 }
 

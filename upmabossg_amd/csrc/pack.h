@@ -40,10 +40,15 @@ Blob pack_blob(const CompiledModel& cm, const RateTables& rt, bool rate32, bool 
 // the kernel.
 inline uint64_t stage_row(int nw) { return 3ull * (uint64_t)nw + 4ull; }
 
-// Window count, fixed-point exponents and discrete step count of a run of n trajectories.
+// Window count, fixed-point exponents and discrete step count of a run of n trajectories, and
+// for the error columns the exponents FQ, FQT of the sums of squared per-trajectory window
+// totals and the scales turning a total into the trajectory's fraction of the window
+// (sq_scale) or its window-average TH (sq_scale_th).
 struct RunScales {
     int W = 0, FP = 0, FT = 0;
     uint64_t S = 0, nsteps = 0;
+    int FQ = 0, FQT = 0;
+    double sq_scale = 1.0, sq_scale_th = 1.0;
 };
 RunScales run_scales(const CompiledModel& cm, uint64_t n);
 

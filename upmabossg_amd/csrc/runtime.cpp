@@ -145,6 +145,16 @@ struct Desc {
     uint64_t cell_mask;
     void* lane_state;  // lanes' state in HBM (MBG_STATE_HBM)
     uint32_t lane_stride, pad2_;
+    // error columns (MBG_ERRORS): sums of squares, the lanes' window lists, their overflow count
+    void* prob2;
+    void* th2;
+    void* cell_vals2;
+    void* err_s;
+    void* err_v;
+    void* err_miss;
+    double sq_scale, sq_scale_th;
+    int FQ, FQT;
+    uint32_t err_cap, err_stride;
 };
 
 using mbg_desc_ptr = const void*;
@@ -152,13 +162,13 @@ using mbg_desc_ptr = const void*;
 struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 2, scan = 4, wsum_reg = 0, rate32 = 0,
         multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0, th_lds = 0, cells_lds = 0, dep_cap = 0,
-        state_hbm = 0;
+        state_hbm = 0, errors = 0;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
-                        sumw, compact, wsmad, sparse, prog, th_lds, cells_lds, dep_cap, state_hbm) <
+                        sumw, compact, wsmad, sparse, prog, th_lds, cells_lds, dep_cap, state_hbm, errors) <
                std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in, o.dep_batch, o.scan,
                         o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact, o.wsmad, o.sparse, o.prog, o.th_lds,
-                        o.cells_lds, o.dep_cap, o.state_hbm);
+                        o.cells_lds, o.dep_cap, o.state_hbm, o.errors);
     }
 };
 
@@ -187,6 +197,7 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_CELLS_LDS " << sh.cells_lds << "\n";
     os << "#define MBG_DEP_CAP " << sh.dep_cap << "\n";
     os << "#define MBG_STATE_HBM " << sh.state_hbm << "\n";
+    os << "#define MBG_ERRORS " << sh.errors << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
     os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
     return os.str();
@@ -233,7 +244,7 @@ struct DevBuf {
 
 struct RunBuffers {
     DevBuf<uint64_t> blob, rtab, stage, prob, th, fin, fp, fpc, summ, work, flips, cells, cell_vals, cell_miss, cell_out,
-        lane_state;
+        lane_state, prob2, th2, cell_vals2, err_s, err_v, err_miss;
     DevBuf<float> istate;
 };
 
@@ -267,6 +278,7 @@ struct SweepClass {
     int blocks_per_cu = 1;
     int wmax = 1;  // largest window count of the class's variants (per-wave entropy partials)
     DevBuf<uint64_t> act;  // the lanes' active sets in HBM (MBG_STATE_HBM 2), shared by the class
+    DevBuf<uint64_t> err_s, err_v, err_miss;  // error columns: the lanes' window lists, overflows
 };
 
 struct SweepState {
@@ -314,6 +326,10 @@ struct mbg_ctx {
     std::vector<uint32_t> cell_w;
     std::vector<uint64_t> cell_s, cell_v, fin_s, fin_v;
     uint64_t cell_cap = 0;
+    // error columns of the last run (when asked for): sums of squares, window-major; the
+    // sparse form follows cell_v
+    std::vector<uint64_t> prob2, th2, cell_v2;
+    uint32_t err_cap = 8;  // entries of each lane's window list to start the next run with
 };
 
 namespace {
@@ -635,7 +651,7 @@ constexpr uint64_t kMaxCells = 1ull << 30;
 // window followed by per-window sorts on up to 16 host threads; the keys of window W are the
 // final states.  The next sparse run starts from twice the size when more than half the slots
 // were used (short probe sequences).
-void read_cells(mbg_ctx* c, const Module& mod, uint64_t cap, int W, int K, hipStream_t s) {
+void read_cells(mbg_ctx* c, const Module& mod, uint64_t cap, int W, int K, hipStream_t s, bool squares) {
     RunBuffers& B = c->buf;
     const unsigned blocks = (unsigned)((cap + 255) / 256);
     uint64_t n = 0;
@@ -652,23 +668,26 @@ void read_cells(mbg_ctx* c, const Module& mod, uint64_t cap, int W, int K, hipSt
         hip_check(hipStreamSynchronize(s), "mbg_cells_count");
     }
     if (n > cap) throw Error(MBG_ERR_HIP, "sparse window table: inconsistent slot count");
-    std::vector<uint64_t> keys(n), vals(n);
+    std::vector<uint64_t> keys(n), vals(n), vals2(squares ? n : 0);
     if (n) {
-        B.cell_out.alloc(2 * n + 1);
-        hip_check(hipMemsetAsync(B.cell_out.p + 2 * n, 0, 8, s), "hipMemsetAsync");
+        B.cell_out.alloc(3 * n + 1);
+        hip_check(hipMemsetAsync(B.cell_out.p + 3 * n, 0, 8, s), "hipMemsetAsync");
         const void* kp = B.cells.p;
         const void* vp = B.cell_vals.p;
+        const void* vp2 = squares ? (const void*)B.cell_vals2.p : nullptr;
         void* ok = B.cell_out.p;
         void* ov = B.cell_out.p + n;
-        void* on = B.cell_out.p + 2 * n;
+        void* ov2 = B.cell_out.p + 2 * n;
+        void* on = B.cell_out.p + 3 * n;
         uint64_t capv = cap, outv = n;
-        void* args[] = {(void*)&kp, (void*)&vp, &capv, &ok, &ov, &on, &outv};
+        void* args[] = {(void*)&kp, (void*)&vp, (void*)&vp2, &capv, &ok, &ov, &ov2, &on, &outv};
         hip_check(hipModuleLaunchKernel(mod.cells_compact, blocks, 1, 1, 256, 1, 1, 0, s, args, nullptr),
                   "launch mbg_cells_compact");
         uint64_t got = 0;
         hip_check(hipMemcpyAsync(&got, on, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
         hip_check(hipMemcpyAsync(keys.data(), ok, n * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
         hip_check(hipMemcpyAsync(vals.data(), ov, n * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        if (squares) hip_check(hipMemcpyAsync(vals2.data(), ov2, n * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
         hip_check(hipStreamSynchronize(s), "mbg_cells_compact");
         if (got != n) throw Error(MBG_ERR_HIP, "sparse window table: compaction count mismatch");
     }
@@ -676,10 +695,11 @@ void read_cells(mbg_ctx* c, const Module& mod, uint64_t cap, int W, int K, hipSt
     std::vector<uint64_t> off((size_t)W + 2, 0);
     for (uint64_t i = 0; i < n; i++) off[(keys[i] >> K) + 1]++;
     for (int b = 0; b <= W; b++) off[b + 1] += off[b];
-    std::vector<std::pair<uint64_t, uint64_t>> kv(n);
+    // (key, value, square) sorted by key; without squares the third member is 0
+    std::vector<std::tuple<uint64_t, uint64_t, uint64_t>> kv(n);
     {
         std::vector<uint64_t> pos(off.begin(), off.end() - 1);
-        for (uint64_t i = 0; i < n; i++) kv[pos[keys[i] >> K]++] = {keys[i], vals[i]};
+        for (uint64_t i = 0; i < n; i++) kv[pos[keys[i] >> K]++] = {keys[i], vals[i], squares ? vals2[i] : 0};
     }
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> pool;
@@ -693,16 +713,18 @@ void read_cells(mbg_ctx* c, const Module& mod, uint64_t cap, int W, int K, hipSt
     c->cell_w.resize(nc);
     c->cell_s.resize(nc);
     c->cell_v.resize(nc);
+    c->cell_v2.assign(squares ? nc : 0, 0);
     c->fin_s.resize(nf);
     c->fin_v.resize(nf);
     for (uint64_t i = 0; i < nc; i++) {
-        c->cell_w[i] = (uint32_t)(kv[i].first >> K);
-        c->cell_s[i] = kv[i].first & smask;
-        c->cell_v[i] = kv[i].second;
+        c->cell_w[i] = (uint32_t)(std::get<0>(kv[i]) >> K);
+        c->cell_s[i] = std::get<0>(kv[i]) & smask;
+        c->cell_v[i] = std::get<1>(kv[i]);
+        if (squares) c->cell_v2[i] = std::get<2>(kv[i]);
     }
     for (uint64_t i = 0; i < nf; i++) {
-        c->fin_s[i] = kv[nc + i].first & smask;
-        c->fin_v[i] = kv[nc + i].second;
+        c->fin_s[i] = std::get<0>(kv[nc + i]) & smask;
+        c->fin_v[i] = std::get<1>(kv[nc + i]);
     }
     c->cell_cap = n * 2 > cap ? std::min(cap * 2, kMaxCells) : cap;
 }
@@ -800,6 +822,8 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     }
     sh.sparse = sparse ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
+    const bool errors = p->error_columns != 0;
+    sh.errors = errors ? 1 : 0;
     const Blob blob = pack_blob(cm, rt, sh.rate32 != 0, sh.compact != 0);
     const size_t state_smem = lane_lds_bytes(cm.NW, sh.block, sh.wsum_reg, sh.state_hbm) +
                               (sh.tables_lds ? blob.lay.bytes : 0);
@@ -809,13 +833,13 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     // window: the hottest atomics of a run), and for small dense tables the whole window
     // table and the final states, as the reference does below 10 KB (window_average_small.cu:78).
     const size_t W_run = (size_t)std::ceil(cm.max_time / cm.time_tick);
-    const size_t th_smem = W_run * 8;
+    const size_t th_smem = W_run * 8 * (errors ? 2 : 1);  // with error columns, the squares too
     sh.th_lds = env_int("MBG_TUNE_TH_LDS", 1) != 0 && W_run <= 4096 &&
                         state_smem + th_smem <= (size_t)c->lds_per_block
                     ? 1
                     : 0;
     const size_t cells_smem = sparse ? 0 : (W_run + 1) * ((size_t)1 << cm.K) * 8;
-    sh.cells_lds = sh.th_lds && !sparse && env_int("MBG_TUNE_CELLS_LDS", 1) != 0 && cells_smem <= (128u << 10) &&
+    sh.cells_lds = sh.th_lds && !sparse && !errors && env_int("MBG_TUNE_CELLS_LDS", 1) != 0 && cells_smem <= (128u << 10) &&
                            state_smem + th_smem + cells_smem <= (size_t)c->lds_per_block
                        ? 1
                        : 0;
@@ -909,6 +933,21 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     D.FT = FT;
     D.sigma = rt.sigma;
     D.compat = p->compat_batch_resume ? 1 : 0;
+    // Error columns: each lane lists its current window's (state, time) entries in a lane-
+    // interleaved buffer (one column per thread of the largest grid), err_cap entries long to
+    // start with (what the last run needed) and four times longer after an overflow; the run is
+    // deterministic, so the repeat gives the same tables.
+    uint32_t err_cap = errors ? c->err_cap : 0;
+    const uint64_t err_stride = grid_max * (uint64_t)sh.block;
+    if (errors) {
+        if (err_stride > 0xffffffffull) throw Error(MBG_ERR_LIMIT, "grid too large for the error columns");
+        const RunScales rsq = run_scales(cm, n);
+        D.FQ = rsq.FQ;
+        D.FQT = rsq.FQT;
+        D.sq_scale = rsq.sq_scale;
+        D.sq_scale_th = rsq.sq_scale_th;
+        D.err_stride = (uint32_t)err_stride;
+    }
 
     EventPair ev;
     hipEvent_t e0 = ev.a, e1 = ev.b;
@@ -924,6 +963,21 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
             B.cells.fill(cell_cap, 0xff, s);  // every key ~0: empty
             B.cell_vals.zero(cell_cap, s);
             B.cell_miss.zero(1, s);
+        }
+        if (errors) {
+            B.prob2.zero((size_t)W * S, s);
+            B.th2.zero((size_t)W, s);
+            if (sparse) B.cell_vals2.zero(cell_cap, s);
+            B.err_s.alloc((size_t)err_cap * err_stride);
+            B.err_v.alloc((size_t)err_cap * err_stride);
+            B.err_miss.zero(1, s);
+            D.prob2 = B.prob2.p;
+            D.th2 = B.th2.p;
+            D.cell_vals2 = sparse ? B.cell_vals2.p : nullptr;
+            D.err_s = B.err_s.p;
+            D.err_v = B.err_v.p;
+            D.err_miss = B.err_miss.p;
+            D.err_cap = err_cap;
         }
         D.prob = d_prob.p;
         D.th = d_th.p;
@@ -957,15 +1011,31 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
             hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
             sim_ms += ms;
         }
-        if (!sparse) break;
-        uint64_t miss = 0;
-        hip_check(hipMemcpy(&miss, B.cell_miss.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
-        if (miss == 0) break;
-        if (cell_cap >= kMaxCells)
-            throw Error(MBG_ERR_LIMIT, "sparse window table full: more distinct (window, state) cells than " +
-                                           std::to_string(kMaxCells / 2));
-        cell_cap = std::min<uint64_t>(cell_cap * 4, kMaxCells);  // repeat the run with a larger table
+        bool again = false;
+        if (sparse) {
+            uint64_t miss = 0;
+            hip_check(hipMemcpy(&miss, B.cell_miss.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
+            if (miss) {
+                if (cell_cap >= kMaxCells)
+                    throw Error(MBG_ERR_LIMIT, "sparse window table full: more distinct (window, state) cells than " +
+                                                   std::to_string(kMaxCells / 2));
+                cell_cap = std::min<uint64_t>(cell_cap * 4, kMaxCells);  // repeat the run with a larger table
+                again = true;
+            }
+        }
+        if (errors) {
+            uint64_t miss = 0;
+            hip_check(hipMemcpy(&miss, B.err_miss.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
+            if (miss) {
+                if (err_cap >= 4096)
+                    throw Error(MBG_ERR_LIMIT, "error columns: a trajectory visits more than 4,096 projected states in one window");
+                err_cap *= 4;  // repeat the run with longer window lists
+                again = true;
+            }
+        }
+        if (!again) break;
     }
+    if (errors) c->err_cap = err_cap;
 
     c->prob.resize((size_t)W * S);  // filled from the device by the first getter that needs it
     c->prob_on_host = false;
@@ -974,7 +1044,18 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     hip_check(hipMemcpy(c->th.data(), d_th.p, c->th.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     if (S) hip_check(hipMemcpy(c->fin.data(), d_fin.p, c->fin.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     c->sparse = sparse;
-    if (sparse) read_cells(c, mod, cell_cap, W, cm.K, s);
+    if (sparse) read_cells(c, mod, cell_cap, W, cm.K, s, errors);
+    c->prob2.clear();
+    c->th2.clear();
+    if (!sparse) c->cell_v2.clear();
+    if (errors) {
+        c->th2.resize(W);
+        hip_check(hipMemcpy(c->th2.data(), B.th2.p, (size_t)W * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (!sparse) {
+            c->prob2.resize((size_t)W * S);
+            hip_check(hipMemcpy(c->prob2.data(), B.prob2.p, c->prob2.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        }
+    }
     uint64_t nfp = 0, flips = 0;
     hip_check(hipMemcpy(&nfp, d_fpc.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
     hip_check(hipMemcpy(&flips, d_flips.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
@@ -1004,6 +1085,9 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     I.lds_bytes = (int)smem;
     I.blocks_per_cu = blocks_per_cu;
     I.state_hbm = sh.state_hbm;
+    I.error_columns = errors ? 1 : 0;
+    I.sq_exponent = errors ? D.FQ : 0;
+    I.th_sq_exponent = errors ? D.FQT : 0;
     c->marg_fn = mod.marginals;
     c->have = true;
 }
@@ -1175,7 +1259,8 @@ int word_class(int nw) {
 }
 
 // Variant sweep: all variants in one HIP graph launch.
-void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
+void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int depth = 0) {
+    const bool errors = (flags & MBG_SWEEP_ERROR_COLUMNS) != 0;
     const auto t0 = std::chrono::steady_clock::now();
     if (c->sweep) {  // return the previous sweep's buffers to the pool
         for (auto& V : c->sweep->vars) c->variant_pool.push_back(std::move(V));
@@ -1224,6 +1309,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             sh.wsmad = env_int("MBG_TUNE_WSMAD", 1) != 0 ? 1 : 0;
             sh.wsum_reg = default_wsum_reg(nwc);
             sh.sumw = default_sumw(nwc, sh.wsum_reg);
+            sh.errors = errors ? 1 : 0;
         }
         SweepClass& C = *sw->classes[it->second];
         C.members.push_back(v);
@@ -1260,7 +1346,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
                 wm = std::max(wm, (int)std::ceil(cmv.max_time / cmv.time_tick));
             }
             int waves2 = 0;
-            const int block2 = act_hbm_block(c, C.sh.nw, 0, (size_t)wm * 8, &waves2);
+            const int block2 = act_hbm_block(c, C.sh.nw, 0, (size_t)wm * 8 * (errors ? 2 : 1), &waves2);
             if (block2 && waves2 > best_waves) {
                 C.sh.state_hbm = 2;
                 C.sh.block = block2;
@@ -1273,7 +1359,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             wmax = std::max(wmax, (int)std::ceil(cmv.max_time / cmv.time_tick));
         }
         const size_t state_lds = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg, C.sh.state_hbm);
-        const size_t th_lds = (size_t)(C.sh.block / 64) * (size_t)wmax * 8;
+        const size_t th_lds = (size_t)(C.sh.block / 64) * (size_t)wmax * 8 * (errors ? 2 : 1);  // + squares
         C.sh.th_lds = env_int("MBG_TUNE_TH_LDS", 1) != 0 && wmax <= 4096 &&
                               state_lds + th_lds <= (size_t)c->lds_per_block &&
                               cu_lds / (state_lds + th_lds) == cu_lds / state_lds
@@ -1299,6 +1385,12 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, C.sim, C.sh.block, C.smem) != hipSuccess || nb < 1) nb = 1;
         C.blocks_per_cu = nb;
         if (C.sh.state_hbm == 2) C.act.alloc((size_t)c->cus * (size_t)nb * (size_t)C.sh.block * (size_t)C.sh.nw);
+        if (errors) {
+            const size_t cols = (size_t)c->cus * (size_t)nb * (size_t)C.sh.block;
+            C.err_s.alloc((size_t)c->err_cap * cols);
+            C.err_v.alloc((size_t)c->err_cap * cols);
+            C.err_miss.zero(1, main);
+        }
     }
     // shared per-sweep buffers: counters and the marginal jobs of every variant
     SweepShared& SS = c->sweep_shared;
@@ -1328,6 +1420,10 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             B.fin.zero((size_t)rs.S, main);
             B.fp.alloc((size_t)fp_cap * V.NWc);
             B.work.zero(1, main);
+            if (errors) {
+                B.prob2.zero((size_t)rs.W * rs.S, main);
+                B.th2.zero((size_t)rs.W, main);
+            }
             if (cm.K > 0) {
                 job_of[v] = (int)jobs.size();
                 jobs.push_back(MargJob{B.prob.p, marg_total, rs.W, cm.K});
@@ -1372,6 +1468,19 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
                 D.lane_state = C.act.p;
                 D.lane_stride = (uint32_t)((uint64_t)c->cus * (uint64_t)C.blocks_per_cu * (uint64_t)C.sh.block);
             }
+            if (errors) {
+                D.prob2 = B.prob2.p;
+                D.th2 = B.th2.p;
+                D.err_s = C.err_s.p;
+                D.err_v = C.err_v.p;
+                D.err_miss = C.err_miss.p;
+                D.sq_scale = rs.sq_scale;
+                D.sq_scale_th = rs.sq_scale_th;
+                D.FQ = rs.FQ;
+                D.FQT = rs.FQT;
+                D.err_cap = c->err_cap;
+                D.err_stride = (uint32_t)((uint64_t)c->cus * (uint64_t)C.blocks_per_cu * (uint64_t)C.sh.block);
+            }
             mbg_run_info& I = V.info;
             I = mbg_run_info{};
             I.windows = rs.W;
@@ -1390,6 +1499,9 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
             I.lds_bytes = (int)C.smem;
             I.blocks_per_cu = C.blocks_per_cu;
             I.state_hbm = C.sh.state_hbm;
+            I.error_columns = errors ? 1 : 0;
+            I.sq_exponent = errors ? rs.FQ : 0;
+            I.th_sq_exponent = errors ? rs.FQT : 0;
             descs.push_back(D);
         }
         C.descs.alloc(descs.size());
@@ -1502,6 +1614,22 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv) {
         SI.transitions += flips;
     }
     c->sweep = std::move(sw);
+    if (errors) {
+        // a lane's window list overflowed: repeat the sweep with lists four times longer (the
+        // sweep is deterministic, so the repeat gives the same tables)
+        uint64_t miss = 0;
+        for (auto& Cp : c->sweep->classes) {
+            uint64_t m = 0;
+            hip_check(hipMemcpy(&m, Cp->err_miss.p, 8, hipMemcpyDeviceToHost), "hipMemcpy");
+            miss += m;
+        }
+        if (miss) {
+            if (c->err_cap >= 4096 || depth > 6)
+                throw Error(MBG_ERR_LIMIT, "error columns: a trajectory visits more than 4,096 projected states in one window");
+            c->err_cap *= 4;
+            run_variants_impl(c, vs, nv, flags, depth + 1);
+        }
+    }
 }
 
 const VariantRun& sweep_variant(const mbg_ctx* c, int v) {
@@ -1514,7 +1642,7 @@ const VariantRun& sweep_variant(const mbg_ctx* c, int v) {
 struct VariantHost {
     const VariantRun& V;
     mbg::PinnedU64 prob;
-    std::vector<uint64_t> th, fin;
+    std::vector<uint64_t> th, fin, prob2, th2;
 
     VariantHost(const mbg_ctx* c, const VariantRun& v) : V(v) {
         hip_check(hipSetDevice(c->device), "hipSetDevice");
@@ -1525,6 +1653,12 @@ struct VariantHost {
         hip_check(hipMemcpy(prob.data(), V.buf.prob.p, W * S * 8, hipMemcpyDeviceToHost), "hipMemcpy");
         hip_check(hipMemcpy(th.data(), V.buf.th.p, W * 8, hipMemcpyDeviceToHost), "hipMemcpy");
         hip_check(hipMemcpy(fin.data(), V.buf.fin.p, S * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (V.info.error_columns) {
+            prob2.resize(W * S);
+            th2.resize(W);
+            hip_check(hipMemcpy(prob2.data(), V.buf.prob2.p, W * S * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+            hip_check(hipMemcpy(th2.data(), V.buf.th2.p, W * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        }
     }
     mbg::Results view() const {
         mbg::Results r;
@@ -1532,6 +1666,10 @@ struct VariantHost {
         r.prob = &prob;
         r.th = &th;
         r.fin = &fin;
+        if (V.info.error_columns) {
+            r.prob2 = &prob2;
+            r.th2 = &th2;
+        }
         r.fixed_states = &V.fixed_states;
         r.fixed_counts = &V.fixed_counts;
         return r;
@@ -1571,6 +1709,11 @@ mbg::Results results_view(const mbg_ctx* c) {
         r.cell_v = &c->cell_v;
         r.fin_s = &c->fin_s;
         r.fin_v = &c->fin_v;
+    }
+    if (c->info.error_columns) {
+        r.th2 = &c->th2;
+        if (c->sparse) r.cell_v2 = &c->cell_v2;
+        else r.prob2 = &c->prob2;
     }
     r.fixed_states = &c->fixed_states;
     r.fixed_counts = &c->fixed_counts;
@@ -1906,6 +2049,50 @@ mbg_status mbg_get_probtraj(const mbg_ctx* c, double* prob, double* th) {
     });
 }
 
+mbg_status mbg_get_probtraj_errors(const mbg_ctx* c, double* err_prob, double* err_th) {
+    if (!c) return MBG_ERR_ARG;
+    if (!c->have || !c->info.error_columns) return MBG_ERR_NOT_BUILT;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        if (err_prob && c->sparse) throw Error(MBG_ERR_LIMIT, kSparseMessage);
+        const mbg::Results r = results_view(c);
+        if (err_prob)
+            for (size_t i = 0; i < c->prob.size(); i++) err_prob[i] = mbg::window_prob_error(r, i);
+        if (err_th)
+            for (size_t w = 0; w < c->th.size(); w++) err_th[w] = mbg::window_th_error(r, w);
+    });
+}
+
+mbg_status mbg_get_probtraj_squares_raw(const mbg_ctx* c, uint64_t* prob_sq, uint64_t* th_sq) {
+    if (!c) return MBG_ERR_ARG;
+    if (!c->have || !c->info.error_columns) return MBG_ERR_NOT_BUILT;
+    if (prob_sq && c->sparse) {
+        const_cast<mbg_ctx*>(c)->err = kSparseMessage;
+        return MBG_ERR_LIMIT;
+    }
+    if (prob_sq) std::memcpy(prob_sq, c->prob2.data(), c->prob2.size() * 8);
+    if (th_sq) std::memcpy(th_sq, c->th2.data(), c->th2.size() * 8);
+    return MBG_OK;
+}
+
+mbg_status mbg_get_probtraj_cell_squares(const mbg_ctx* c, uint64_t* squares, size_t cap, size_t* n) {
+    if (!c) return MBG_ERR_ARG;
+    if (!c->have || !c->info.error_columns) return MBG_ERR_NOT_BUILT;
+    size_t k = 0;
+    auto put = [&](uint64_t q) {
+        if (k < cap && squares) squares[k] = q;
+        k++;
+    };
+    if (c->sparse) {
+        for (uint64_t q : c->cell_v2) put(q);
+    } else {
+        ensure_prob(c);
+        for (size_t i = 0; i < c->prob.size(); i++)
+            if (c->prob[i]) put(c->prob2[i]);
+    }
+    if (n) *n = k;
+    return MBG_OK;
+}
+
 mbg_status mbg_get_final_states(const mbg_ctx* c, uint64_t* counts) {
     if (!c || !counts) return MBG_ERR_ARG;
     if (!c->have) return MBG_ERR_NOT_BUILT;
@@ -2043,10 +2230,14 @@ mbg_status mbg_get_marginals(const mbg_ctx* c, double* out) {
 }
 
 mbg_status mbg_run_variants(mbg_ctx* c, const mbg_variant* variants, int count) {
-    if (!c || !variants || count <= 0) return MBG_ERR_ARG;
+    return mbg_run_variants_ex(c, variants, count, 0);
+}
+
+mbg_status mbg_run_variants_ex(mbg_ctx* c, const mbg_variant* variants, int count, int flags) {
+    if (!c || !variants || count <= 0 || (flags & ~MBG_SWEEP_ERROR_COLUMNS)) return MBG_ERR_ARG;
     return guarded(c, [&] {
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        run_variants_impl(c, variants, count);
+        run_variants_impl(c, variants, count, flags);
     });
 }
 
@@ -2069,6 +2260,17 @@ mbg_status mbg_get_variant_probtraj_raw(const mbg_ctx* c, int v, uint64_t* prob,
         const size_t W = (size_t)V.info.windows, S = (size_t)1 << V.info.readouts;
         if (prob) hip_check(hipMemcpy(prob, V.buf.prob.p, W * S * 8, hipMemcpyDeviceToHost), "hipMemcpy");
         if (th) hip_check(hipMemcpy(th, V.buf.th.p, W * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+mbg_status mbg_get_variant_probtraj_squares_raw(const mbg_ctx* c, int v, uint64_t* prob_sq, uint64_t* th_sq) {
+    if (!c) return MBG_ERR_ARG;
+    return guarded(const_cast<mbg_ctx*>(c), [&] {
+        const VariantRun& V = sweep_variant(c, v);
+        if (!V.info.error_columns) throw Error(MBG_ERR_NOT_BUILT, "the sweep did not accumulate the error columns");
+        const size_t W = (size_t)V.info.windows, S = (size_t)1 << V.info.readouts;
+        if (prob_sq) hip_check(hipMemcpy(prob_sq, V.buf.prob2.p, W * S * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (th_sq) hip_check(hipMemcpy(th_sq, V.buf.th2.p, W * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     });
 }
 
