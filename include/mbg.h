@@ -221,6 +221,16 @@ typedef struct mbg_sweep_info {
     double setup_ms;     /* host table building, uploads and graph construction */
 } mbg_sweep_info;
 mbg_status mbg_get_sweep_info(const mbg_ctx* c, mbg_sweep_info* out);
+/* Word class i (0 <= i < classes) of the last sweep: its launch shape and what it ran.  No
+ * reference counterpart (the reference runs one model per process). */
+typedef struct mbg_sweep_class_info {
+    int words, discrete, variants;
+    uint64_t trajectories, transitions;
+    double kernel_ms;    /* the class kernel's span inside the graph (first block start to last end) */
+    int block, blocks_per_cu, state_hbm, wsum_reg, sumw, dep_batch, dep_cap, scan, th_lds;
+    int tuned;           /* 1: the shape came from mbg_autotune_variants */
+} mbg_sweep_class_info;
+mbg_status mbg_get_sweep_class_info(const mbg_ctx* c, int i, mbg_sweep_class_info* out);
 /* Per-variant results of the last sweep (same layouts as the single-run getters). */
 mbg_status mbg_get_variant_info(const mbg_ctx* c, int v, mbg_run_info* out);
 mbg_status mbg_get_variant_probtraj_raw(const mbg_ctx* c, int v, uint64_t* prob, uint64_t* th);

@@ -26,16 +26,32 @@ HOST_SOURCES = ["front.cpp", "model.cpp", "pack.cpp"]
 CXX = os.environ.get("CXX", "g++")
 
 
-def _flags(asan: bool):
+MSAN_CXX = "/opt/rocm/llvm/bin/clang++"
+last_stderr = ""  # the last emulated run's diagnostics (sanitizer reports)
+
+
+def _cxx(msan: bool = False):
+    return MSAN_CXX if msan else CXX
+
+
+def _flags(asan: bool, msan: bool = False):
     f = ["-O2", "-g", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wno-unknown-pragmas",
-         "-I" + B.CSRC, "-I" + HERE, "-I" + BUILD, "-DMBG_ACT_GROUPS_PROBE"]
+         "-I" + B.CSRC, "-I" + HERE, "-I" + BUILD]
     if asan:
         f += ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer"]
+    if msan:
+        # MemorySanitizer: a branch, address or output that depends on a never-written value
+        # (a local, LDS, a device buffer) is reported with the origin of that value
+        f += ["-fsanitize=memory", "-fsanitize-memory-track-origins=2", "-fsanitize-recover=memory",
+              "-fno-omit-frame-pointer", "-fsanitize-ignorelist=" + os.path.join(HERE, "msan_ignore.txt")]
+    # extra flags for the emulated kernel (for instance -ftrivial-auto-var-init=pattern, which
+    # gives every uninitialised local a fixed non-zero pattern)
+    f += os.environ.get("EMU_EXTRA_FLAGS", "").split()
     return f
 
 
-def _host_objects(asan: bool):
-    out = os.path.join(BUILD, "host_asan" if asan else "host")
+def _host_objects(asan: bool, msan: bool = False):
+    out = os.path.join(BUILD, "host_msan" if msan else "host_asan" if asan else "host")
     os.makedirs(out, exist_ok=True)
     objs, procs = [], []
     for s in HOST_SOURCES:
@@ -45,7 +61,7 @@ def _host_objects(asan: bool):
         deps = [src] + [os.path.join(B.CSRC, h) for h in ("front.h", "model.h", "pack.h")]
         if os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
             continue
-        procs.append(subprocess.Popen([CXX, *_flags(asan), "-c", src, "-o", obj], stdout=subprocess.PIPE,
+        procs.append(subprocess.Popen([_cxx(msan), *_flags(asan, msan), "-c", src, "-o", obj], stdout=subprocess.PIPE,
                                       stderr=subprocess.PIPE, text=True))
     for p in procs:
         o, e = p.communicate()
@@ -54,7 +70,7 @@ def _host_objects(asan: bool):
     return objs
 
 
-def build(shape: dict, asan: bool = False) -> str:
+def build(shape: dict, asan: bool = False, msan: bool = False) -> str:
     """Executable emulating the kernel compiled for `shape` (keyword arguments of
     upmabossg_amd.build.kernel_source); cached by the source text and flags."""
     os.makedirs(BUILD, exist_ok=True)
@@ -63,15 +79,17 @@ def build(shape: dict, asan: bool = False) -> str:
         dev = f.read()
     with open(os.path.join(HERE, "emu_run.cpp")) as f:
         drv = f.read()
-    key = hashlib.sha1((src + dev + drv + repr(_flags(asan))).encode()).hexdigest()[:16]
-    exe = os.path.join(BUILD, f"emu_{key}{'_asan' if asan else ''}")
-    objs = _host_objects(asan)
+    key = hashlib.sha1((src + dev + drv + repr(_flags(asan, msan))).encode()).hexdigest()[:16]
+    exe = os.path.join(BUILD, f"emu_{key}{'_asan' if asan else ''}{'_msan' if msan else ''}")
+    # under MemorySanitizer the model parser and packer stay uninstrumented (their outputs are
+    # marked written before the kernels run), so that only the kernel code is checked
+    objs = _host_objects(asan, False)
     if os.path.exists(exe) and os.path.getmtime(exe) >= max(os.path.getmtime(o) for o in objs):
         return exe
     kfile = os.path.join(BUILD, f"k_{key}.h")
     with open(kfile, "w") as f:
         f.write(src)
-    cmd = [CXX, *_flags(asan), f'-DEMU_KERNEL="k_{key}.h"', os.path.join(HERE, "emu_run.cpp"), *objs,
+    cmd = [_cxx(msan), *_flags(asan, msan), f'-DEMU_KERNEL="k_{key}.h"', os.path.join(HERE, "emu_run.cpp"), *objs,
            "-o", exe + ".tmp"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode:
@@ -164,10 +182,13 @@ def run(exe: str, variants, grid: int = 1, compat: int = 0, cell_cap: int = 1 <<
     mu, md, istate arrays.  Returns one dict of raw tables per variant."""
     with tempfile.TemporaryDirectory() as td:
         job = _job(td, variants, grid, compat, cell_cap, stride_extra, err_cap)
-        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0")
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0",
+                   MSAN_OPTIONS="halt_on_error=0")
         r = subprocess.run([exe, job, td], capture_output=True, text=True, timeout=timeout, env=env)
         if r.returncode:
             raise RuntimeError(f"emulated run failed ({r.returncode}):\n{r.stdout[-4000:]}\n{r.stderr[-4000:]}")
+        global last_stderr
+        last_stderr = r.stderr
         out = []
         for i in range(len(variants)):
             def rd(name):

@@ -40,6 +40,23 @@
 #define EMU_ASAN 0
 #endif
 
+// Under MemorySanitizer: memory the kernel must write before reading it (LDS, device
+// buffers) is marked never-written, and inputs read from files (written by uninstrumented
+// library code) are marked written.
+#if defined(__has_feature)
+#if __has_feature(memory_sanitizer)
+#include <sanitizer/msan_interface.h>
+#define EMU_MSAN 1
+#endif
+#endif
+#ifdef EMU_MSAN
+inline void emu_poison(const void* p, size_t n) { __msan_poison(p, n); }
+inline void emu_unpoison(const void* p, size_t n) { __msan_unpoison(p, n); }
+#else
+inline void emu_poison(const void*, size_t) {}
+inline void emu_unpoison(const void*, size_t) {}
+#endif
+
 namespace emu {
 
 struct Dim3 {
@@ -147,6 +164,7 @@ inline void launch(Dim3 grid, Dim3 block, size_t smem_bytes, bool interleave, st
         for (size_t b = 0; b < blocks.size(); b++) {
             blocks[b].idx = Dim3{all[g0 + b].first, all[g0 + b].second, 0};
             blocks[b].lds.assign((smem_bytes + 7) / 8, 0xA5A5A5A5A5A5A5A5ull);  // LDS is not zeroed
+            emu_poison(blocks[b].lds.data(), blocks[b].lds.size() * 8);
             for (unsigned i = 0; i < nthreads; i++) {
                 Thread& t = threads[b * nthreads + i];
                 t.tid = Dim3{i, 0, 0};
@@ -298,5 +316,19 @@ inline T atomicCAS(T* p, T cmp, T v) {
     if (old == cmp) *p = v;
     return old;
 }
+template <class T>
+inline T atomicMin(T* p, T v) {
+    const T old = *p;
+    if (v < old) *p = v;
+    return old;
+}
+template <class T>
+inline T atomicMax(T* p, T v) {
+    const T old = *p;
+    if (v > old) *p = v;
+    return old;
+}
+// the device's constant-rate clock: not modelled (the kernels only time themselves with it)
+inline unsigned long long wall_clock64() { return 0; }
 using std::max;
 using std::min;

@@ -36,7 +36,9 @@ std::string slurp(const std::string& path) {
     if (!f) throw std::runtime_error("cannot read " + path);
     std::stringstream ss;
     ss << f.rdbuf();
-    return ss.str();
+    std::string s = ss.str();
+    emu_unpoison(s.data(), s.size());
+    return s;
 }
 
 std::vector<float> floats(const std::string& path) {
@@ -84,9 +86,11 @@ int main(int argc, char** argv) {
         uint64_t cell_cap = 1 << 16, stride_extra = 0, err_cap = 8;
         std::string line;
         while (std::getline(job, line)) {
+            emu_unpoison(line.data(), line.size());
             std::istringstream ls(line);
             std::string key;
             if (!(ls >> key)) continue;
+            emu_unpoison(key.data(), key.size());
             if (key == "grid") ls >> grid;
             else if (key == "compat") ls >> compat;
             else if (key == "cell_cap") ls >> cell_cap;
@@ -97,6 +101,12 @@ int main(int argc, char** argv) {
                 std::string bnd, cfg, mu, md, is;
                 ls >> bnd >> cfg >> V->n >> V->count >> V->first_id >> V->seed >> V->vid >> mu >> md >> is;
                 if (!ls) throw std::runtime_error("bad variant line: " + line);
+                for (std::string* t : {&bnd, &cfg, &mu, &md, &is}) emu_unpoison(t->data(), t->size());
+                emu_unpoison(&V->n, sizeof(V->n));
+                emu_unpoison(&V->count, sizeof(V->count));
+                emu_unpoison(&V->first_id, sizeof(V->first_id));
+                emu_unpoison(&V->seed, sizeof(V->seed));
+                emu_unpoison(&V->vid, sizeof(V->vid));
                 mbg::parse_model(slurp(bnd), slurp(cfg), V->pm);
                 V->cm = mbg::compile_model(V->pm);
                 const std::vector<float> fu = floats(mu), fd = floats(md), fi = floats(is);
@@ -117,12 +127,15 @@ int main(int argc, char** argv) {
 #if MBG_STATE_HBM
         lane_state.assign((size_t)stride * (MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW,
                           0x5A5A5A5A5A5A5A5Aull);
+        emu_poison(lane_state.data(), lane_state.size() * 8);
 #endif
         // error columns: the lanes' window lists (one column per grid thread) and their overflow count
         std::vector<u64> err_s, err_v, err_miss(1, 0);
 #if MBG_ERRORS
         err_s.assign((size_t)err_cap * (grid * (uint64_t)MBG_BLOCK), 0x3C3C3C3C3C3C3C3Cull);
         err_v.assign((size_t)err_cap * (grid * (uint64_t)MBG_BLOCK), 0x3C3C3C3C3C3C3C3Cull);
+        emu_poison(err_s.data(), err_s.size() * 8);
+        emu_poison(err_v.data(), err_v.size() * 8);
 #endif
         int wmax = 1;
         for (auto& Vp : vars) {
@@ -140,10 +153,12 @@ int main(int argc, char** argv) {
             wmax = std::max(wmax, V.rs.W);
             const uint64_t S = MBG_SPARSE ? 0 : V.rs.S;
             V.stage.assign((size_t)V.count * mbg::stage_row(MBG_NW), 0);
+            emu_poison(V.stage.data(), V.stage.size() * 8);  // written by the initial-state pass
             V.prob.assign((size_t)V.rs.W * S, 0);
             V.th.assign((size_t)V.rs.W, 0);
             V.fin.assign((size_t)S, 0);
             V.fp.assign((size_t)V.count * MBG_NW, 0);
+            emu_poison(V.fp.data(), V.fp.size() * 8);  // fixed-point records, written before read
             V.summ.assign(MBG_TRACE ? (size_t)V.count * 4 : 1, 0);
             V.counters.assign(3, 0);  // fixed points, transitions, work queue
             V.istate = V.rt.istate;
@@ -213,6 +228,32 @@ int main(int argc, char** argv) {
             D.err_stride = (u32)(grid * (uint64_t)MBG_BLOCK);
 #endif
         }
+        // everything the host prepared counts as written; what the kernels must write first
+        // (staging rows, fixed-point records, lanes' state in HBM, window lists) does not
+        for (auto& Vp : vars) {
+            Variant& V = *Vp;
+            auto clean = [](auto& vec) { emu_unpoison(vec.data(), vec.size() * sizeof(vec[0])); };
+            clean(V.blob.words);
+            clean(V.rt.rtab);
+            clean(V.istate);
+            clean(V.prob);
+            clean(V.th);
+            clean(V.fin);
+            clean(V.summ);
+            clean(V.counters);
+            clean(V.cells);
+            clean(V.cellv);
+            clean(V.cell_miss);
+            clean(V.prob2);
+            clean(V.th2);
+            clean(V.cellv2);
+            emu_unpoison(&V.D, sizeof(V.D));
+            emu_poison(V.stage.data(), V.stage.size() * 8);
+            emu_poison(V.fp.data(), V.fp.size() * 8);
+        }
+        emu_unpoison(err_miss.data(), err_miss.size() * 8);
+        emu_unpoison(&wmax, sizeof(wmax));
+        emu_unpoison(&grid, sizeof(grid));
         const size_t th_cells = MBG_ERRORS ? 2 : 1;  // window entropies, then their squares
         const size_t lane_words =
             MBG_STATE_HBM == 1 ? 0 : (size_t)(MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
@@ -242,11 +283,14 @@ int main(int argc, char** argv) {
             bx = std::max<uint64_t>(bx, (Vp->count + 63) / 64);
         }
         const mbg_desc* dp = descs.data();
+        emu_unpoison(descs.data(), descs.size() * sizeof(mbg_desc));
+        emu_unpoison(&dp, sizeof(dp));
         emu::launch({(unsigned)bx, (unsigned)descs.size(), 1}, {256, 1, 1}, 0, false, [&] { mbg_init_rates_multi(dp); });
         const size_t smem_words = lane_words + (MBG_TH_LDS ? (size_t)(MBG_BLOCK / 64) * th_cells * (size_t)wmax : 0);
         const int nv = (int)descs.size();
+        emu_unpoison(&nv, sizeof(nv));
         emu::launch({grid, 1, 1}, {MBG_BLOCK, 1, 1}, smem_words * 8, true,
-                    [&] { mbg_simulate_multi(dp, nv, wmax); });
+                    [&] { mbg_simulate_multi(dp, nv, wmax, nullptr); });
 #endif
         for (size_t v = 0; v < vars.size(); v++) {
             Variant& X = *vars[v];

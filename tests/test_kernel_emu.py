@@ -7,9 +7,10 @@ layout is checked on the CPU, before and independently of any GPU run.
 
 The 1,100-node variant in a 20-word sweep class with the active sets in HBM and two-word rate
 sum groups is the layout whose GPU run once counted 226,839 transitions against the oracle's
-227,473; emulated, the same source, shape, block, grid and draws give the oracle's numbers
-exactly (also under AddressSanitizer), so that discrepancy does not come from the kernel's
-source semantics.
+227,473 with kernels compiled at -O3; emulated, the same source, shape, block, grid and draws
+give the oracle's numbers exactly (also under AddressSanitizer), and the GPU gives them too
+with the -O1 code the runtime now builds (tests/test_gpu_codegen.py covers the shapes whose
+-O3 code miscounted).
 """
 import numpy as np
 import pytest
@@ -105,11 +106,15 @@ def test_sweep_class_of_the_gpu_miscount():
     _run([v], grid=1, asan=True, trace=False, nw=20, block=768, multi=1, wsum_reg=1, sumw=2, state_hbm=2)
 
 
-def test_sweep_class_several_variants():
+@pytest.mark.parametrize("state_hbm,wsum_reg,sumw", [(2, 1, 1), (1, 1, 2), (1, 2, 3)])
+def test_sweep_class_several_variants(state_hbm, wsum_reg, sumw):
+    """A sweep class of three variants with the active sets (2) or the whole lane state (1) in
+    HBM; with the whole state in HBM also with two-word register sum groups and with two-level
+    sums."""
     vs = [_net(n, seed=s, count=c) for n, s, c in [(300, 4, 70), (420, 5, 40), (500, 6, 90)]]
     for k, v in enumerate(vs):
         v.update(seed=20 + k, vid=k)
-    _run(vs, grid=3, trace=False, nw=8, block=128, multi=1, wsum_reg=1, state_hbm=2)
+    _run(vs, grid=3, trace=False, nw=8, block=128, multi=1, wsum_reg=wsum_reg, sumw=sumw, state_hbm=state_hbm)
 
 
 def test_wide_model_state_in_hbm_two_level_sums():

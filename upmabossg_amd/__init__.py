@@ -89,6 +89,13 @@ class _SweepInfo(C.Structure):
                 ("transitions", C.c_uint64), ("graph_ms", C.c_double), ("setup_ms", C.c_double)]
 
 
+class _SweepClassInfo(C.Structure):
+    _fields_ = [("words", C.c_int), ("discrete", C.c_int), ("variants", C.c_int), ("trajectories", C.c_uint64),
+                ("transitions", C.c_uint64), ("kernel_ms", C.c_double)] + \
+               [(k, C.c_int) for k in ("block", "blocks_per_cu", "state_hbm", "wsum_reg", "sumw", "dep_batch",
+                                       "dep_cap", "scan", "th_lds", "tuned")]
+
+
 _LIB = None
 
 
@@ -140,6 +147,7 @@ def lib():
         "mbg_run_variants_ex": (C.c_int, [vp, C.POINTER(_Variant), C.c_int, C.c_int]),
         "mbg_get_variant_probtraj_squares_raw": (C.c_int, [vp, C.c_int, u64p, u64p]),
         "mbg_get_sweep_info": (C.c_int, [vp, C.POINTER(_SweepInfo)]),
+        "mbg_get_sweep_class_info": (C.c_int, [vp, C.c_int, C.POINTER(_SweepClassInfo)]),
         "mbg_get_variant_info": (C.c_int, [vp, C.c_int, C.POINTER(_RunInfo)]),
         "mbg_get_variant_probtraj_raw": (C.c_int, [vp, C.c_int, u64p, u64p]),
         "mbg_get_variant_marginals": (C.c_int, [vp, C.c_int, f64p]),
@@ -503,6 +511,15 @@ class Context:
         i = _SweepInfo()
         self._check(lib().mbg_get_sweep_info(self._h, C.byref(i)))
         return i
+
+    def sweep_classes(self):
+        """Per word class of the last sweep: its launch shape, counts and kernel span, as dicts."""
+        out = []
+        for i in range(self.sweep_info().classes):
+            ci = _SweepClassInfo()
+            self._check(lib().mbg_get_sweep_class_info(self._h, i, C.byref(ci)))
+            out.append({k: getattr(ci, k) for k, _ in _SweepClassInfo._fields_})
+        return out
 
     def variant_info(self, v) -> _RunInfo:
         i = _RunInfo()

@@ -137,14 +137,9 @@ typedef unsigned int sidx;
 #ifndef MBG_STATE_HBM
 #define MBG_STATE_HBM 0
 #endif
-#if MBG_STATE_HBM == 1 && MBG_MULTI
-#error "the whole lane state in HBM is a layout of single runs"
-#endif
-// The active sets alone in HBM with sum groups of several words: correct in the host emulation
-// of this source, but one device run of that layout miscounted (cause not identified), so the
-// runtime never compiles it; the host emulation defines MBG_ACT_GROUPS_PROBE to keep checking it.
-#if MBG_STATE_HBM == 2 && (MBG_WSUM_REG != 1 || MBG_SUMW != 1) && !defined(MBG_ACT_GROUPS_PROBE)
-#error "the active sets alone in HBM need one register rate sum per word (at most 16 words)"
+// The active sets alone in HBM keep x alone in LDS: the rate sums must be in registers.
+#if MBG_STATE_HBM == 2 && MBG_WSUM_REG != 1
+#error "the active sets alone in HBM need register rate sums"
 #endif
 
 // A node record decoded into registers (either table layout).
@@ -929,6 +924,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                             if (MBG_WSUM_REG == 2 && w >= MBG_NW) continue;  // short last group
                             LANE_X(w) = st[w];
                             LANE_A(w) = st[MBG_NW + w];
+
                             gs += st[2 * MBG_NW + w];
 #if MBG_WSUM_REG == 2
                             LANE_W(w) = st[2 * MBG_NW + w];
@@ -962,6 +958,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         }
         if (__ballot(have) == 0ull) break;
         if (!have) continue;
+
         if (hold) {
             hold = false;
             continue;
@@ -1388,8 +1385,12 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
 // empty; the variants' tables stay in HBM.
 // With MBG_TH_LDS each wave sums its current variant's window entropies in its own LDS slice
 // of wmax cells (after the lanes' state) and flushes them when it moves to the next variant.
-extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const mbg_desc* descs, int nv, int wmax) {
+// clk (may be null): the class's first block start and last block end on the device's
+// constant-rate clock, by atomic min / max (the class's kernel time inside a graph).
+extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const mbg_desc* descs, int nv, int wmax,
+                                                                           u64* clk) {
     MBG_DYN_LDS(smem);
+    if (clk && threadIdx.x == 0) atomicMin(clk, (u64)wall_clock64());
     const u32 waves_total = gridDim.x * (MBG_BLOCK / 64);
     const u32 wave = blockIdx.x * (MBG_BLOCK / 64) + (threadIdx.x >> 6);
     const int start = (int)(((u64)wave * (u64)nv) / waves_total);
@@ -1397,7 +1398,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
     // the slice holds the variant's W entropies, then (MBG_ERRORS) their W squares
     const u32 lid = threadIdx.x & 63u;
     const u32 th_cells = (MBG_ERRORS ? 2u : 1u);
-    u64* thw = smem + (MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK +
+    u64* thw = smem + (MBG_STATE_HBM == 1 ? 0 : MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK +
                (threadIdx.x >> 6) * th_cells * (u32)wmax;
 #else
     (void)wmax;
@@ -1413,7 +1414,9 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
 #endif
-        simulate_variant(D, D.blob, smem, thw);
+        // the lanes' state in LDS, or with MBG_STATE_HBM 1 in the class's HBM buffer (every
+        // descriptor of the class names the same one)
+        simulate_variant(D, D.blob, MBG_STATE_HBM == 1 ? D.lane_state : smem, thw);
 #if MBG_TH_LDS
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();  // every lane of the wave has left the variant's loop
@@ -1424,6 +1427,10 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
             else atomicAdd(D.th2 + (c - (u32)D.W), x);
         }
 #endif
+    }
+    if (clk) {
+        __syncthreads();
+        if (threadIdx.x == 0) atomicMax(clk + 1, (u64)wall_clock64());
     }
 }
 #endif

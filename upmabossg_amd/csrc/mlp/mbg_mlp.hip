@@ -579,6 +579,9 @@ mbg_status mbg_mlp_train(mbg_mlp* m, int epochs, int batch, float lr, uint64_t s
         const size_t steps = m->nset / batch;
         double sum = 0.0;
         for (int e = 0; e < epochs; e++) {
+            // the previous epoch's upload of perm (pageable memory, possibly still pending)
+            // must finish before perm is shuffled again
+            if (e > 0) check(hipStreamSynchronize(m->s), "epoch");
             std::shuffle(perm.begin(), perm.end(), rng);
             check(hipMemcpyAsync(m->idx.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, m->s), "hipMemcpyAsync");
             sum = 0.0;

@@ -277,8 +277,12 @@ struct SweepClass {
     size_t smem = 0;
     int blocks_per_cu = 1;
     int wmax = 1;  // largest window count of the class's variants (per-wave entropy partials)
-    DevBuf<uint64_t> act;  // the lanes' active sets in HBM (MBG_STATE_HBM 2), shared by the class
+    DevBuf<uint64_t> act;  // the lanes' active sets (MBG_STATE_HBM 2) or whole state (1) in HBM
     DevBuf<uint64_t> err_s, err_v, err_miss;  // error columns: the lanes' window lists, overflows
+    double work = 0;         // launch-order estimate: trajectories x nodes x max_time over the members
+    double kernel_ms = 0;    // the class kernel's span in the last graph (first block start to last end)
+    uint64_t trajectories = 0, transitions = 0;
+    bool tuned = false;      // the shape came from mbg_autotune_variants
 };
 
 struct SweepState {
@@ -292,6 +296,7 @@ struct SweepShared {
     DevBuf<uint64_t> counters;  // [variant][2]: fixed-point count, transitions
     DevBuf<MargJob> jobs;
     DevBuf<uint64_t> marg;      // all variants' marginals, [variant][W][K]
+    DevBuf<uint64_t> clk;       // [class][2]: first block start, last block end (device clock)
 };
 
 }  // namespace
@@ -300,6 +305,7 @@ struct mbg_ctx {
     int device = 0;
     int cus = 0;
     int lds_per_block = 65536;  // device limit on (static + dynamic) LDS per workgroup
+    int wall_khz = 0;           // rate of the device's constant clock (wall_clock64)
     hipStream_t stream = nullptr;
     std::string err;
     std::map<Shape, Module> modules;
@@ -341,9 +347,21 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "mbg_kernel.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
         throw Error(MBG_ERR_COMPILE, "hiprtcCreateProgram failed");
-    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
-                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    // -O1: with -O2 and -O3 some shapes of these kernels (a 177-node model in blocks of 512 or
+    // 1024 threads, the 4-, 24- and 28-word sweep classes) gave wrong transition counts that
+    // changed from run to run on gfx950, while the kernel source runs bit-exact in the host
+    // emulation (also under AddressSanitizer and MemorySanitizer); -O1 code of the same shapes
+    // is bit-exact and deterministic
+    std::vector<const char*> opts = {"--offload-arch=gfx950", "-O1", "-std=c++17", "-ffp-contract=off",
+                                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    // MBG_RTC_OPTS: extra compiler options, space-separated (code-generation experiments)
+    std::vector<std::string> extra;
+    if (const char* e = std::getenv("MBG_RTC_OPTS")) {
+        std::istringstream is(e);
+        for (std::string o; is >> o;) extra.push_back(o);
+    }
+    for (const std::string& o : extra) opts.push_back(o.c_str());
+    const hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
     size_t log_size = 0;
     hiprtcGetProgramLogSize(prog, &log_size);
     std::string log(log_size, '\0');
@@ -408,9 +426,10 @@ int default_sumw(int nw, int wsum_reg) {
     if (!wsum_reg) return 1;
     if (wsum_reg == 2) return (nw + 15) / 16;  // at most 16 groups, the last one possibly short
     int sw = env_int("MBG_TUNE_SUMW", nw > 16 ? 2 : 1);
+    const int max_groups = env_int("MBG_TUNE_MAX_GROUPS", 16);
     if (sw < 1) sw = 1;
     while (sw > 1 && nw % sw) sw /= 2;
-    while (nw / sw > 16 && nw % (sw * 2) == 0) sw *= 2;
+    while (nw / sw > max_groups && nw % (sw * 2) == 0) sw *= 2;
     return sw;
 }
 
@@ -496,15 +515,14 @@ int layout_block(const mbg_ctx* c, int nw, int wsum_reg, uint32_t blob_bytes, in
 // and per_wave_bytes per wave.  At most 12 waves per block: launch bounds of 768 threads leave
 // the step loop about 170 VGPRs (3 waves per SIMD), while 16 would cap it at 128 and spill, so
 // a CU holds at most 12 waves of this kernel whatever the LDS allows.  0 if nothing fits.
-// The layout is limited to one register rate sum per word (at most 16 words).  With groups of
-// two words (a 1,100-node variant in the 20-word sweep class) one GPU run of the sweep test
-// counted 226,839 transitions against the oracle's 227,473.  The kernel source of that exact
-// shape, block, grid and draws, compiled for the host and run in the CPU emulation
-// (tests/kernel_emu), gives the oracle's results bit for bit and stays in bounds under
-// AddressSanitizer: the kernel's semantics are correct there, and the difference arises in code
-// generation or execution on the device, which is not identified.  So the layout is not used
-// with sum groups of more than one word.
-bool act_hbm_allowed(int wsum_reg, int sumw) { return wsum_reg == 1 && sumw == 1; }
+// The layout needs the rate sums in registers (x alone stays in LDS): one sum per word up to
+// 16 words, or per group of two words up to 32.  (With groups of two, a sweep of the 20-word
+// class once miscounted with kernels compiled at -O3; at -O1, see get_module, that sweep and
+// single runs of 1,300-2,000-node models in this layout are bit-exact.)
+bool act_hbm_allowed(int wsum_reg, int sumw) {
+    (void)sumw;
+    return wsum_reg == 1;
+}
 
 int act_hbm_block(const mbg_ctx* c, int nw, size_t fixed_bytes, size_t per_wave_bytes, int* waves_out) {
     const size_t cu_lds = 160 * 1024;
@@ -1007,6 +1025,7 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
             launch(mod.sim, grid, sh.block, smem, s, D, "launch mbg_simulate");
             hip_check(hipEventRecord(e1, s), "hipEventRecord");
             hip_check(hipStreamSynchronize(s), "kernel execution");
+
             float ms = 0.f;
             hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
             sim_ms += ms;
@@ -1313,6 +1332,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
         }
         SweepClass& C = *sw->classes[it->second];
         C.members.push_back(v);
+        C.work += (double)V->n * (double)cm.N * (double)cm.max_time;
         C.sh.generic |= cm.any_generic ? 1 : 0;
         C.sh.prog |= cm.any_prog ? 1 : 0;
         C.sh.max_in = std::max(C.sh.max_in, cm.max_inputs);
@@ -1336,6 +1356,10 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
             }
         }
         if (!best_waves) throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
+        if (const int bq = env_int("MBG_TUNE_SWEEP_BLOCK", 0)) {
+            if (bq % 64 == 0 && bq <= 1024 && lane_lds_bytes(C.sh.nw, bq, C.sh.wsum_reg) <= (size_t)c->lds_per_block)
+                C.sh.block = bq;
+        }
         // the active sets in HBM where that keeps more waves resident (up to 12 per CU, which
         // leave the step loop its VGPRs), as in single runs
         // leave room for the per-wave window-entropy partials
@@ -1352,6 +1376,11 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
                 C.sh.block = block2;
             }
         }
+        // the whole lane state in HBM (one buffer per class)
+        if (env_int("MBG_TUNE_SWEEP_STATE_HBM", 0) == 1 && C.sh.state_hbm != 2) {
+            C.sh.state_hbm = 1;
+            C.sh.block = 256;
+        }
         // per-wave window-entropy partials in LDS when they fit without costing occupancy
         int wmax = 1;
         for (int v : C.members) {
@@ -1360,9 +1389,11 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
         }
         const size_t state_lds = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg, C.sh.state_hbm);
         const size_t th_lds = (size_t)(C.sh.block / 64) * (size_t)wmax * 8 * (errors ? 2 : 1);  // + squares
+        // (with the state in HBM: when 16 waves' slices fit in a CU's LDS)
+        const bool th_fits = state_lds ? cu_lds / (state_lds + th_lds) == cu_lds / state_lds
+                                       : th_lds * (size_t)(16 / (C.sh.block / 64)) <= cu_lds;
         C.sh.th_lds = env_int("MBG_TUNE_TH_LDS", 1) != 0 && wmax <= 4096 &&
-                              state_lds + th_lds <= (size_t)c->lds_per_block &&
-                              cu_lds / (state_lds + th_lds) == cu_lds / state_lds
+                              state_lds + th_lds <= (size_t)c->lds_per_block && th_fits
                           ? 1
                           : 0;
         C.wmax = wmax;
@@ -1383,8 +1414,17 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
         C.smem = state_lds + (C.sh.th_lds ? th_lds : 0);
         int nb = 0;
         if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, C.sim, C.sh.block, C.smem) != hipSuccess || nb < 1) nb = 1;
+        if (C.sh.state_hbm == 1) {
+            // keep the class's lane state within about half of the Infinity Cache
+            const uint64_t lane_bytes = (uint64_t)(C.sh.wsum_reg == 1 ? 2 : 3) * (uint64_t)C.sh.nw * 8;
+            const uint64_t fit = (uint64_t)env_int("MBG_TUNE_STATE_MB", 128) * (1ull << 20) /
+                                 (lane_bytes * (uint64_t)C.sh.block * (uint64_t)c->cus);
+            nb = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nb, fit));
+        }
         C.blocks_per_cu = nb;
-        if (C.sh.state_hbm == 2) C.act.alloc((size_t)c->cus * (size_t)nb * (size_t)C.sh.block * (size_t)C.sh.nw);
+        if (C.sh.state_hbm)
+            C.act.alloc((size_t)c->cus * (size_t)nb * (size_t)C.sh.block * (size_t)C.sh.nw *
+                        (C.sh.state_hbm == 2 ? 1 : C.sh.wsum_reg == 1 ? 2 : 3));
         if (errors) {
             const size_t cols = (size_t)c->cus * (size_t)nb * (size_t)C.sh.block;
             C.err_s.alloc((size_t)c->err_cap * cols);
@@ -1399,19 +1439,27 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
     std::vector<int> job_of(nv, -1);
     uint64_t marg_total = 0;
     int w_max = 0;
-    // per variant: tables and buffers
+    // per variant: tables and buffers.  The host sources of the asynchronous uploads (packed
+    // tables, descriptors) are kept until the uploads have completed: a copy from pageable
+    // memory may read its source after hipMemcpyAsync returns.
+    std::vector<std::vector<uint64_t>> keep_blobs;
+    std::vector<std::vector<Desc>> keep_descs;
+    keep_blobs.reserve((size_t)nv);
+    keep_descs.reserve(sw->classes.size());
     for (auto& Cp : sw->classes) {
         SweepClass& C = *Cp;
-        std::vector<Desc> descs;
+        keep_descs.emplace_back();
+        std::vector<Desc>& descs = keep_descs.back();
         for (int v : C.members) {
             VariantRun& V = *sw->vars[v];
             const mbg::CompiledModel& cm = V.m->cm;
             const mbg::RateTables& rt = rts[v];
-            const Blob blob = pack_blob(cm, rt, C.sh.rate32 != 0, C.sh.compact != 0);
+            Blob blob = pack_blob(cm, rt, C.sh.rate32 != 0, C.sh.compact != 0);
             const RunScales rs = run_scales(cm, V.n);
             const uint64_t fp_cap = V.n;
             RunBuffers& B = V.buf;
             B.blob.upload(blob.words, main);
+            keep_blobs.push_back(std::move(blob.words));
             B.rtab.upload(rt.rtab, main);
             B.istate.upload(rt.istate, main);
             B.stage.alloc((size_t)V.n * stage_row(V.NWc));
@@ -1463,7 +1511,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
             D.FP = rs.FP;
             D.FT = rs.FT;
             D.sigma = rt.sigma;
-            if (C.sh.state_hbm == 2) {
+            if (C.sh.state_hbm) {
                 // one column per thread of the class's largest grid (cus x blocks_per_cu blocks)
                 D.lane_state = C.act.p;
                 D.lane_stride = (uint32_t)((uint64_t)c->cus * (uint64_t)C.blocks_per_cu * (uint64_t)C.sh.block);
@@ -1514,7 +1562,14 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
                   "hipMemcpyAsync");
         SS.marg.alloc(marg_total);
     }
-    hip_check(hipStreamSynchronize(main), "sweep set-up");
+    {
+        // class clocks: start slots at the maximum (atomic min), end slots at 0 (atomic max)
+        std::vector<uint64_t> clk0(2 * sw->classes.size());
+        for (size_t i = 0; i < clk0.size(); i++) clk0[i] = i % 2 ? 0ull : ~0ull;
+        SS.clk.alloc(clk0.size());
+        hip_check(hipMemcpyAsync(SS.clk.p, clk0.data(), clk0.size() * 8, hipMemcpyHostToDevice, main), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(main), "sweep set-up");
+    }
     // one stream per class, forked from and joined to the main stream inside the graph
     while (c->class_streams.size() < sw->classes.size()) {
         hipStream_t st;
@@ -1523,13 +1578,21 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
     }
     CaptureScope cap;
     cap.stream = main;
-    cap.events.assign(sw->classes.size() + 1, nullptr);
+    cap.events.assign(2 * sw->classes.size() + 2, nullptr);
     std::vector<hipEvent_t>& evs = cap.events;
     for (auto& e : evs) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipStreamBeginCapture(main, hipStreamCaptureModeGlobal), "hipStreamBeginCapture");
     cap.capturing = true;
     hip_check(hipEventRecord(evs[0], main), "hipEventRecord");
-    for (size_t ci = 0; ci < sw->classes.size(); ci++) {
+    // Every class's initial-state pass first, all of them joined before any class kernel
+    // starts: a pass launched behind persistent class kernels only gets the CUs they leave,
+    // and waited there up to 280 ms.  Then the class kernels, the most work first.
+    const size_t ncls = sw->classes.size();
+    std::vector<size_t> order(ncls);
+    for (size_t ci = 0; ci < ncls; ci++) order[ci] = ci;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return sw->classes[a]->work > sw->classes[b]->work; });
+    for (size_t ci = 0; ci < ncls; ci++) {
         SweepClass& C = *sw->classes[ci];
         hipStream_t st = c->class_streams[ci];
         hip_check(hipStreamWaitEvent(st, evs[0], 0), "hipStreamWaitEvent");
@@ -1552,6 +1615,16 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
                 launch(C.init_readouts, (D.batch + 255) / 256, 256, 0, st, D, "launch mbg_init_readouts");
             }
         }
+        hip_check(hipEventRecord(evs[ncls + 1 + ci], st), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(main, evs[ncls + 1 + ci], 0), "hipStreamWaitEvent");
+    }
+    hip_check(hipEventRecord(evs[2 * ncls + 1], main), "hipEventRecord");
+    const bool serial = env_int("MBG_TUNE_SWEEP_SERIAL", 0) != 0;  // class kernels one after another
+    for (size_t oi = 0; oi < ncls; oi++) {
+        const size_t ci = order[oi];
+        SweepClass& C = *sw->classes[ci];
+        hipStream_t st = serial ? main : c->class_streams[ci];
+        if (!serial) hip_check(hipStreamWaitEvent(st, evs[2 * ncls + 1], 0), "hipStreamWaitEvent");
         uint64_t traj = 0;
         for (int v : C.members) traj += sw->vars[v]->n;
         uint64_t grid = (uint64_t)c->cus * (uint64_t)C.blocks_per_cu;
@@ -1559,12 +1632,15 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
         const mbg_desc_ptr dp = C.descs.p;
         int n_members = (int)C.members.size();
         int wmax_arg = C.wmax;
-        void* args[] = {(void*)&dp, &n_members, &wmax_arg};
+        uint64_t* clk = SS.clk.p + 2 * ci;
+        void* args[] = {(void*)&dp, &n_members, &wmax_arg, (void*)&clk};
         hip_check(hipModuleLaunchKernel(C.sim, (unsigned)grid, 1, 1, (unsigned)C.sh.block, 1, 1, (unsigned)C.smem, st,
                                         args, nullptr),
                   "launch mbg_simulate_multi");
-        hip_check(hipEventRecord(evs[ci + 1], st), "hipEventRecord");
-        hip_check(hipStreamWaitEvent(main, evs[ci + 1], 0), "hipStreamWaitEvent");
+        if (!serial) {
+            hip_check(hipEventRecord(evs[ci + 1], st), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(main, evs[ci + 1], 0), "hipStreamWaitEvent");
+        }
     }
     if (!jobs.empty()) {
         // every variant's readout marginals, reduced from its window table on the device
@@ -1593,8 +1669,20 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
     SI.classes = (int)sw->classes.size();
     SI.graph_ms = ms;
     SI.setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    std::vector<uint64_t> counters((size_t)2 * nv), marg(marg_total);
+    std::vector<uint64_t> counters((size_t)2 * nv), marg(marg_total), clk(2 * sw->classes.size());
     hip_check(hipMemcpy(counters.data(), SS.counters.p, counters.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    hip_check(hipMemcpy(clk.data(), SS.clk.p, clk.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    for (size_t ci = 0; ci < sw->classes.size(); ci++) {
+        SweepClass& C = *sw->classes[ci];
+        C.kernel_ms = clk[2 * ci + 1] > clk[2 * ci] && c->wall_khz > 0
+                          ? (double)(clk[2 * ci + 1] - clk[2 * ci]) / (double)c->wall_khz
+                          : 0.0;
+        C.trajectories = C.transitions = 0;
+        for (int v : C.members) {
+            C.trajectories += sw->vars[v]->n;
+            C.transitions += counters[2 * (size_t)v + 1];
+        }
+    }
     if (marg_total) hip_check(hipMemcpy(marg.data(), SS.marg.p, marg_total * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     for (int v = 0; v < nv; v++) {
         VariantRun& V = *sw->vars[v];
@@ -1968,6 +2056,7 @@ mbg_status mbg_create(int device, mbg_ctx** out) {
     if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return MBG_ERR_HIP;
     if (hipDeviceGetAttribute(&c->lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
         return MBG_ERR_HIP;
+    if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->wall_khz = 0;
     if (hipStreamCreate(&c->stream) != hipSuccess) return MBG_ERR_HIP;
     *out = c.release();
     return MBG_OK;
@@ -2245,6 +2334,32 @@ mbg_status mbg_get_sweep_info(const mbg_ctx* c, mbg_sweep_info* out) {
     if (!c || !out) return MBG_ERR_ARG;
     if (!c->sweep) return MBG_ERR_NOT_BUILT;
     *out = c->sweep->info;
+    return MBG_OK;
+}
+
+mbg_status mbg_get_sweep_class_info(const mbg_ctx* c, int i, mbg_sweep_class_info* out) {
+    if (!c || !out) return MBG_ERR_ARG;
+    if (!c->sweep) return MBG_ERR_NOT_BUILT;
+    if (i < 0 || i >= (int)c->sweep->classes.size()) return MBG_ERR_ARG;
+    const SweepClass& C = *c->sweep->classes[(size_t)i];
+    mbg_sweep_class_info& o = *out;
+    o = mbg_sweep_class_info{};
+    o.words = C.sh.nw;
+    o.discrete = C.sh.discrete;
+    o.variants = (int)C.members.size();
+    o.trajectories = C.trajectories;
+    o.transitions = C.transitions;
+    o.kernel_ms = C.kernel_ms;
+    o.block = C.sh.block;
+    o.blocks_per_cu = C.blocks_per_cu;
+    o.state_hbm = C.sh.state_hbm;
+    o.wsum_reg = C.sh.wsum_reg;
+    o.sumw = C.sh.sumw;
+    o.dep_batch = C.sh.dep_batch;
+    o.dep_cap = C.sh.dep_cap;
+    o.scan = C.sh.scan;
+    o.th_lds = C.sh.th_lds;
+    o.tuned = C.tuned ? 1 : 0;
     return MBG_OK;
 }
 
