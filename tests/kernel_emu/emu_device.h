@@ -286,6 +286,9 @@ inline T shfl(T v, int src) {
 #define MBG_OPAQUE_V(x) ((void)0)
 #define MBG_OPAQUE_S(x) ((void)0)
 #define MBG_DYN_LDS(name) u64* name = emu::dyn_lds()
+// four u32 read as one 16-byte access (GNU vector; indexed like the device's ext_vector_type)
+#define MBG_U32X4 1
+typedef unsigned int mbg_u32x4 __attribute__((vector_size(16)));
 
 #define threadIdx (emu::W().cur->tid)
 #define blockIdx (emu::W().cur->blk->idx)

@@ -65,6 +65,10 @@
 typedef unsigned int u32;
 typedef unsigned short u16;
 typedef unsigned long long u64;
+// four u32 read as one 16-byte access
+#ifndef MBG_U32X4
+typedef u32 mbg_u32x4 __attribute__((ext_vector_type(4)));
+#endif
 
 // Pointers into memory that no kernel writes while it runs (the model tables).
 #ifndef CONST_AS
@@ -184,6 +188,7 @@ struct mbg_layout {
     u32 gen_mask;  // u64[NW]: nodes with a generic rate table
     u32 bytes;     // blob size
     u32 prog;      // u32[]: logic programs of nodes with in_cnt 255 ([length, ops...])
+    u32 onehot;    // u32[16][16] identity: row g holds the factors [i == g] of a rate-sum update
 };
 
 struct mbg_desc {
@@ -327,7 +332,25 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
 #define NSUM (MBG_NW / MBG_SUMW)
 #endif
 #define LANE_S(g) ws[g]
-#if MBG_WSMAD
+#if MBG_WSMAD && MBG_TABLES_LDS
+// every rate change fits an i32: ws[i] += [i == g] * d as one 32 x 32 + 64 multiply-add, the
+// factors [i == g] read as row g of the identity table staged in LDS with the model (four
+// 16-byte reads on the LDS pipe instead of one bit-field extract per sum on the VALU)
+#define WS_GADD(g, d)                                                      \
+    do {                                                                   \
+        const mbg_u32x4* row_ = (const mbg_u32x4*)(tb + D.lay.onehot) + (g) * 4u; \
+        const long long d_ = (long long)(int)(u32)(d);                     \
+        _Pragma("unroll") for (int q_ = 0; q_ < (NSUM + 3) / 4; q_++) {    \
+            const mbg_u32x4 f4_ = row_[q_];                                \
+            const u32 f_[4] = {f4_[0], f4_[1], f4_[2], f4_[3]};            \
+            _Pragma("unroll") for (int e_ = 0; e_ < 4; e_++) {             \
+                const int i_ = q_ * 4 + e_;                                \
+                if (i_ < NSUM)                                             \
+                    ws[i_] = (u64)((long long)ws[i_] + (long long)(int)f_[e_] * d_); \
+            }                                                              \
+        }                                                                  \
+    } while (0)
+#elif MBG_WSMAD
 // every rate change fits an i32: ws[i] += [i == g] * d as one 32 x 32 + 64 multiply-add, the
 // factor [i == g] as one bit-field extract of the one-hot word (hidden from the compiler, which
 // would otherwise rebuild each factor as a compare and a select)
@@ -760,6 +783,16 @@ __device__ __forceinline__ void flush_cell(const mbg_desc& D, u64* part, int w, 
 #endif
 }
 
+// (u64)rint(x) for 0 <= x < 2^52: x + 2^52 lies in [2^52, 2^53), where the spacing of doubles
+// is 1, so the addition rounds x to an integer (to nearest, ties to even, as rint does) and
+// the low 52 bits of the sum are that integer: one FP64 add and one 32-bit subtract instead of
+// rint and a double-to-u64 conversion.  `small` (wave-uniform) says every value of the run
+// stays below 2^52; otherwise the general conversion is used.
+__device__ __forceinline__ u64 rint_u64(double x, bool small) {
+    if (small) return __builtin_bit_cast(u64, x + 4503599627370496.0) - 0x4330000000000000ull;
+    return (u64)rint(x);
+}
+
 #define DEP_BATCH MBG_DEP_BATCH
 
 // Simulates trajectories of one model (descriptor D) on this wave until its work queue is
@@ -862,6 +895,13 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #endif
     Lane L{D, tb, lane, tid};
     const u32 key0 = (u32)D.seed, key1 = (u32)(D.seed >> 32);
+    // every slice's fixed-point time and TH x time stays below 2^51 (a slice is at most one
+    // window, TH at most log2 63 < 8), so their rounding takes the two-instruction path
+#if MBG_DISCRETE
+    const bool acc52 = ldexp(8.0, D.FT) < 2251799813685248.0;
+#else
+    const bool acc52 = ldexp(D.tick, D.FP) < 2251799813685248.0 && ldexp(D.tick * 8.0, D.FT) < 2251799813685248.0;
+#endif
 
     bool have = false, exhausted = false;
     // Step parity is kept equal across the wave: a trajectory starts only on an even loop
@@ -1104,7 +1144,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                 acc_s = s;
             }
             acc_p += 1;
-            acc_th += (u64)rint(ldexp(th_use, D.FT));
+            acc_th += rint_u64(ldexp(th_use, D.FT), acc52);
         }
 #else
         {
@@ -1126,8 +1166,8 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     acc_p = 0;
                     acc_s = s;
                 }
-                acc_p += (u64)rint(ldexp(sl, D.FP));
-                acc_th += (u64)rint(ldexp(th_use * sl, D.FT));
+                acc_p += rint_u64(ldexp(sl, D.FP), acc52);
+                acc_th += rint_u64(ldexp(th_use * sl, D.FT), acc52);
                 w++;
                 b = e;
             }

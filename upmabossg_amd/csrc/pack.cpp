@@ -75,6 +75,10 @@ Blob pack_blob(const CompiledModel& cm, const RateTables& rt, bool rate32, bool 
     b.lay.terms = seg(rt.terms.data(), rt.terms.size() * 8);
     b.lay.gen_mask = seg(cm.gen_mask.data(), cm.gen_mask.size() * 8);
     b.lay.prog = seg(cm.prog.data(), cm.prog.size() * 4);
+    // factors [i == g] of the register rate-sum updates, read as four 16-byte rows
+    uint32_t onehot[16 * 16] = {};
+    for (int g = 0; g < 16; g++) onehot[g * 16 + g] = 1u;
+    b.lay.onehot = seg(onehot, sizeof(onehot));
     bytes.resize((bytes.size() + 15) & ~(size_t)15);
     b.lay.bytes = (uint32_t)bytes.size();
     b.words.resize(bytes.size() / 8);

@@ -16,6 +16,7 @@ namespace mbg {
 // device/mbg_kernel.hip.
 struct Layout {
     uint32_t nodes, rates, deps, inputs, lut, readouts, terms, gen_mask, bytes, prog;
+    uint32_t onehot;  // u32[16][16] identity: row g holds the factors [i == g] of a rate-sum update
 };
 
 // Model tables packed into one blob (16-byte aligned segments).
