@@ -162,6 +162,8 @@ def shape_for(variants, **over):
               prog=int(any(f["prog"] for f in fs)), th_lds=1, cells_lds=0,
               dep_cap=max(4, 2 * db, math.ceil(1.5 * mean)), state_hbm=0)
     sh.update(over)
+    if sh["multi"] and "sweep_xcd" not in over:
+        sh["sweep_xcd"] = 2  # the runtime's default: every wave starts at the class's first variant
     if sh["wsum_reg"] and "wsmad" not in over:
         sh["wsmad"] = int(all(f["wsmad"] for f in fs))
     if not sh["wsum_reg"]:

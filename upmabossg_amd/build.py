@@ -70,7 +70,7 @@ def kernel_source(nw: int, block: int = 64, discrete: int = 0, trace: int = 0, g
                   tables_lds: int = 1, max_in: int = 4, dep_batch: int = 2, scan: int = 4, wsum_reg: int = 0,
                   rate32: int = 0, multi: int = 0, sumw: int = 1, compact: int = 0, wsmad: int = 0,
                   sparse: int = 0, prog: int = 0, th_lds: int = 0, cells_lds: int = 0, dep_cap: int = 0,
-                  state_hbm: int = 0, errors: int = 0) -> str:
+                  state_hbm: int = 0, errors: int = 0, sweep_xcd: int = 0) -> str:
     with open(os.path.join(DEV, "mbg_math.h")) as f:
         math_src = f.read()
     with open(os.path.join(DEV, "mbg_kernel.hip")) as f:
@@ -82,7 +82,7 @@ def kernel_source(nw: int, block: int = 64, discrete: int = 0, trace: int = 0, g
             f"#define MBG_SUMW {sumw}\n#define MBG_COMPACT {compact}\n#define MBG_WSMAD {wsmad}\n"
             f"#define MBG_SPARSE {sparse}\n#define MBG_PROG {prog}\n#define MBG_TH_LDS {th_lds}\n"
             f"#define MBG_CELLS_LDS {cells_lds}\n#define MBG_DEP_CAP {dep_cap}\n#define MBG_STATE_HBM {state_hbm}\n"
-            f"#define MBG_ERRORS {errors}\n"
+            f"#define MBG_ERRORS {errors}\n#define MBG_SWEEP_XCD {sweep_xcd}\n"
             f"#define MBG_HD __device__ __forceinline__\n{math_src}\n{kern_src}\n")
 
 
@@ -108,7 +108,7 @@ def check_kernel(verbose=False):
             errors = 1 if i in (1, 2, 4) else 0  # the error columns' sums: dense, sweep and sparse
             with open(src, "w") as f:
                 f.write(kernel_source(nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp,
-                                      pg, th, cells, cap, hbm, errors))
+                                      pg, th, cells, cap, hbm, errors, multi))
             cmd = [HIPCC, *KERNEL_FLAGS, "-include", "hip/hip_runtime.h", "--cuda-device-only", "-c", src, "-o", src + ".o"]
             if verbose:
                 print("+", " ".join(cmd), flush=True)

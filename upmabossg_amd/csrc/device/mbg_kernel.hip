@@ -26,6 +26,8 @@
 //   MBG_ERRORS      1: also the sums of squares behind the ErrorProba / ErrorTH columns (each
 //                   trajectory's time per projected state and TH x time over each window,
 //                   squared); each lane keeps its current window's (state, time) list in HBM
+//   MBG_SWEEP_XCD   sweep waves start at variant 0, the costliest per trajectory (2), at variants
+//                   grouped by block and XCD (1), or spread per wave (0)
 //   MBG_STATE_HBM   1: the lanes' x, act and word sums in an HBM buffer (lane-interleaved over
 //                   the whole grid) instead of LDS, for models whose per-lane state would leave
 //                   a CU with one or two waves (single runs only); 2: only act in that buffer,
@@ -140,6 +142,9 @@ typedef unsigned int sidx;
 #endif
 #ifndef MBG_STATE_HBM
 #define MBG_STATE_HBM 0
+#endif
+#ifndef MBG_SWEEP_XCD
+#define MBG_SWEEP_XCD 0
 #endif
 // The active sets alone in HBM keep x alone in LDS: the rate sums must be in registers.
 #if MBG_STATE_HBM == 2 && MBG_WSUM_REG != 1
@@ -1431,9 +1436,22 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
                                                                            u64* clk) {
     MBG_DYN_LDS(smem);
     if (clk && threadIdx.x == 0) atomicMin(clk, (u64)wall_clock64());
+#if MBG_SWEEP_XCD == 2
+    // every wave starts at the class's first variant: the host orders them by decreasing work
+    // per trajectory, so the costliest trajectories start first and the run ends on the cheapest
+    const int start = 0;
+#elif MBG_SWEEP_XCD
+    // the waves of a block start at one variant, and the blocks of one XCD (blocks are dispatched
+    // to the 8 XCDs in turn) at consecutive variants of one eighth of the class: an XCD's L2 then
+    // holds the tables of the variants its CUs work on, and a CU's waves share their L1 lines
+    const u32 per_xcd = (gridDim.x + 7u) / 8u;
+    const u32 order = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
+    const int start = (int)(((u64)order * (u64)nv) / (8ull * per_xcd));
+#else
     const u32 waves_total = gridDim.x * (MBG_BLOCK / 64);
     const u32 wave = blockIdx.x * (MBG_BLOCK / 64) + (threadIdx.x >> 6);
     const int start = (int)(((u64)wave * (u64)nv) / waves_total);
+#endif
 #if MBG_TH_LDS
     // the slice holds the variant's W entropies, then (MBG_ERRORS) their W squares
     const u32 lid = threadIdx.x & 63u;
