@@ -80,16 +80,23 @@ def copy_calibration(nbytes=1 << 30):
     hip.hipFree(b)
 
 
-def sweep_bench(ctx, count=256):
+def sweep_bench(ctx, count=256, tune=True):
     from upmabossg_amd import Model, synth
 
     nets = synth.sweep_variants(count)
     models = [Model(n.bnd(), n.cfg()) for n in nets]
     vs = [(m, n.sample_count, 1, v) for v, (m, n) in enumerate(zip(models, nets))]
+    t_tune = time.perf_counter()
+    if tune:
+        ctx.autotune_variants(vs)  # set-up, untimed: per-class layouts measured on these variants
+    t_tune = time.perf_counter() - t_tune
     ctx.run_variants(vs)  # JIT of the word classes, buffers
     t0 = time.perf_counter()
     info = ctx.run_variants(vs)
     wall = time.perf_counter() - t0
+    classes = [{k: c[k] for k in ("words", "variants", "transitions", "kernel_ms", "block", "blocks_per_cu",
+                                  "tables_lds", "state_hbm", "wsum_reg", "sumw", "dep_batch", "dep_cap", "tuned")}
+               for c in ctx.sweep_classes()]
     return {
         "workload": f"sweep: {count} variant networks of {min(n.n for n in nets)}-{max(n.n for n in nets)} nodes, "
                     "rates x log-normal(0, 0.5), 0-3 knockouts, 1k-64k trajectories each, one HIP graph",
@@ -103,6 +110,8 @@ def sweep_bench(ctx, count=256):
         "transitions_per_s": info.transitions / wall,
         "transitions_per_s_graph": info.transitions / (info.graph_ms / 1e3),
         "variants_per_s": count / wall,
+        "autotune_s": t_tune,
+        "classes": classes,
     }
 
 
@@ -263,7 +272,7 @@ def main():
         sweep = None
         if not args.no_sweep:
             try:
-                sweep = sweep_bench(ctx)
+                sweep = sweep_bench(ctx, tune=not args.no_autotune)
             except Exception as e:  # reported, never fatal to the headline line
                 sweep = {"error": f"{type(e).__name__}: {e}"}
             print(f"sweep: {sweep}", file=sys.stderr, flush=True)

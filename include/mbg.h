@@ -213,6 +213,13 @@ mbg_status mbg_run_variants(mbg_ctx* c, const mbg_variant* variants, int count);
  * sums (as mbg_run_params.error_columns does for one run). */
 #define MBG_SWEEP_ERROR_COLUMNS 1
 mbg_status mbg_run_variants_ex(mbg_ctx* c, const mbg_variant* variants, int count, int flags);
+/* Sweep autotuner (no reference counterpart: the reference has no sweeps).  For each word class
+ * that these variants form, runs the class's variants alone under candidate layouts (tables
+ * staged per block in LDS or read from HBM, active sets in HBM or LDS, one or two dependents
+ * per round), keeps the fastest per class, and applies it to later sweeps that form the same
+ * class (mbg_sweep_class_info.tuned).  Results never depend on the layout.  flags as for
+ * mbg_run_variants_ex.  Leaves the last tuning run as the context's last sweep. */
+mbg_status mbg_autotune_variants(mbg_ctx* c, const mbg_variant* variants, int count, int flags);
 
 typedef struct mbg_sweep_info {
     int variants, classes;
@@ -229,6 +236,7 @@ typedef struct mbg_sweep_class_info {
     double kernel_ms;    /* the class kernel's span inside the graph (first block start to last end) */
     int block, blocks_per_cu, state_hbm, wsum_reg, sumw, dep_batch, dep_cap, scan, th_lds;
     int tuned;           /* 1: the shape came from mbg_autotune_variants */
+    int tables_lds;      /* 1: each block stages its current variant's tables in LDS */
 } mbg_sweep_class_info;
 mbg_status mbg_get_sweep_class_info(const mbg_ctx* c, int i, mbg_sweep_class_info* out);
 /* Per-variant results of the last sweep (same layouts as the single-run getters). */

@@ -286,11 +286,23 @@ int main(int argc, char** argv) {
         emu_unpoison(descs.data(), descs.size() * sizeof(mbg_desc));
         emu_unpoison(&dp, sizeof(dp));
         emu::launch({(unsigned)bx, (unsigned)descs.size(), 1}, {256, 1, 1}, 0, false, [&] { mbg_init_rates_multi(dp); });
-        const size_t smem_words = lane_words + (MBG_TH_LDS ? (size_t)(MBG_BLOCK / 64) * th_cells * (size_t)wmax : 0);
+        // tables in LDS: the class's largest blob, then a 16-byte flag, before the lanes' state
+        int tab_words = 0;
+        if (MBG_TABLES_LDS)
+            for (auto& Vp : vars) tab_words = std::max(tab_words, (int)(Vp->blob.lay.bytes / 8));
+        const size_t smem_words = (MBG_TABLES_LDS ? (size_t)tab_words + 2 : 0) + lane_words +
+                                  (MBG_TH_LDS ? (size_t)(MBG_BLOCK / 64) * th_cells * (size_t)wmax : 0);
         const int nv = (int)descs.size();
+        // blocks start at variants spread over the class (block b at variant b mod nv)
+        std::vector<int> starts(grid);
+        for (unsigned b = 0; b < grid; b++) starts[b] = (int)(b % (unsigned)nv);
+        const int* sp = MBG_TABLES_LDS ? starts.data() : nullptr;
         emu_unpoison(&nv, sizeof(nv));
+        emu_unpoison(&tab_words, sizeof(tab_words));
+        emu_unpoison(&sp, sizeof(sp));
+        emu_unpoison(starts.data(), starts.size() * sizeof(int));
         emu::launch({grid, 1, 1}, {MBG_BLOCK, 1, 1}, smem_words * 8, true,
-                    [&] { mbg_simulate_multi(dp, nv, wmax, nullptr); });
+                    [&] { mbg_simulate_multi(dp, nv, wmax, nullptr, tab_words, sp); });
 #endif
         for (size_t v = 0; v < vars.size(); v++) {
             Variant& X = *vars[v];

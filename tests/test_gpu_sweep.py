@@ -84,3 +84,52 @@ def test_sweep_refuses_more_than_20_readouts(gpu_ctx):
     with pytest.raises(MbgError) as e:
         gpu_ctx.run_variants([(Model(net.bnd(), net.cfg()), 100, 1, 0)])
     assert e.value.status == 2 and "at most 20" in str(e.value)
+
+
+@pytest.mark.parametrize("tables", ["0", "1"])
+def test_sweep_tables_in_lds_or_hbm_match_oracle(gpu_ctx, monkeypatch, tables):
+    """Every class with its blocks' tables staged in LDS (all waves of a block on one variant,
+    blocks starting at work-proportional variants), and every class with the tables read from
+    HBM: the variants' tables equal their oracle replays bit for bit."""
+    monkeypatch.setenv("MBG_TUNE_SWEEP_TABLES", tables)
+    specs = _variants()[:9]
+    models = []
+    for b, c, count, var in specs:
+        m = Model(b, c)
+        m.set_variant(*var)
+        models.append(m)
+    print(f"case: sweep of {len(models)} variants, tables in LDS {tables}", flush=True)
+    gpu_ctx.run_variants([(m, count, 3 + v, v) for v, (m, (_, _, count, _)) in enumerate(zip(models, specs))])
+    staged = [c["tables_lds"] for c in gpu_ctx.sweep_classes()]
+    # (a class whose largest tables pass 64 KiB reads them from HBM either way)
+    assert (sum(staged) >= len(staged) - 1) if tables == "1" else not any(staged), staged
+    for v, (b, c, count, var) in enumerate(specs):
+        o = OracleModel(b, c)
+        o.set_variant(*var)
+        r = o.run(count=count, seed=3 + v, variant_id=v, threads=16)
+        i, prob, th, fin, fs, fc = gpu_ctx.variant_result(v)
+        assert i.transitions == r.flips, v
+        assert np.array_equal(prob, r.prob) and np.array_equal(th, r.th) and np.array_equal(fin, r.final), v
+        assert np.array_equal(fs, r.fixed_states) and np.array_equal(fc, r.fixed_counts), v
+
+
+def test_autotuned_sweep_gives_identical_results(gpu_ctx):
+    """mbg_autotune_variants measures each class's layouts; the next sweep of the same variants
+    uses the chosen ones (tuned = 1) and gives the untuned sweep's tables bit for bit."""
+    nets = synth.sweep_variants(24, n_hi=1200, traj_lo=500, traj_hi=3000)
+    models = [Model(n.bnd(), n.cfg()) for n in nets]
+    vs = [(m, n.sample_count, 1, v) for v, (m, n) in enumerate(zip(models, nets))]
+    print("case: 24-variant sweep, untuned", flush=True)
+    gpu_ctx.run_variants(vs)
+    before = [gpu_ctx.variant_result(v)[:4] for v in range(len(vs))]
+    assert not any(c["tuned"] for c in gpu_ctx.sweep_classes())
+    print("case: 24-variant sweep, autotune", flush=True)
+    gpu_ctx.autotune_variants(vs)
+    print("case: 24-variant sweep, tuned", flush=True)
+    gpu_ctx.run_variants(vs)
+    assert all(c["tuned"] for c in gpu_ctx.sweep_classes())
+    for v in range(len(vs)):
+        i, prob, th, fin = gpu_ctx.variant_result(v)[:4]
+        assert i.transitions == before[v][0].transitions, v
+        assert np.array_equal(prob, before[v][1]) and np.array_equal(th, before[v][2]), v
+        assert np.array_equal(fin, before[v][3]), v

@@ -117,6 +117,19 @@ def test_sweep_class_several_variants(state_hbm, wsum_reg, sumw):
     _run(vs, grid=3, trace=False, nw=8, block=128, multi=1, wsum_reg=wsum_reg, sumw=sumw, state_hbm=state_hbm)
 
 
+@pytest.mark.parametrize("state_hbm,wsum_reg,sumw", [(2, 1, 1), (0, 1, 1), (0, 2, 3)])
+def test_sweep_class_tables_in_lds(state_hbm, wsum_reg, sumw):
+    """A sweep class whose blocks stage their current variant's tables in LDS (every wave of a
+    block on one variant, blocks moving on together): four variants of different table sizes
+    on three blocks, the blocks starting at different variants, with the active sets in HBM or
+    the lanes' state in LDS (register sums, or two-level sums)."""
+    vs = [_net(n, seed=s, count=c) for n, s, c in [(300, 4, 70), (420, 5, 40), (500, 6, 90), (260, 7, 30)]]
+    for k, v in enumerate(vs):
+        v.update(seed=30 + k, vid=k)
+    _run(vs, grid=3, trace=False, nw=8, block=128, multi=1, tables_lds=1, wsum_reg=wsum_reg, sumw=sumw,
+         state_hbm=state_hbm)
+
+
 def test_wide_model_state_in_hbm_two_level_sums():
     v = _net(2600, count=24, max_time=4.0)
     _run([v], grid=2, block=64, wsum_reg=2, sumw=3, state_hbm=1)

@@ -93,7 +93,7 @@ class _SweepClassInfo(C.Structure):
     _fields_ = [("words", C.c_int), ("discrete", C.c_int), ("variants", C.c_int), ("trajectories", C.c_uint64),
                 ("transitions", C.c_uint64), ("kernel_ms", C.c_double)] + \
                [(k, C.c_int) for k in ("block", "blocks_per_cu", "state_hbm", "wsum_reg", "sumw", "dep_batch",
-                                       "dep_cap", "scan", "th_lds", "tuned")]
+                                       "dep_cap", "scan", "th_lds", "tuned", "tables_lds")]
 
 
 _LIB = None
@@ -145,6 +145,7 @@ def lib():
         "mbg_get_marginals": (C.c_int, [vp, f64p]),
         "mbg_run_variants": (C.c_int, [vp, C.POINTER(_Variant), C.c_int]),
         "mbg_run_variants_ex": (C.c_int, [vp, C.POINTER(_Variant), C.c_int, C.c_int]),
+        "mbg_autotune_variants": (C.c_int, [vp, C.POINTER(_Variant), C.c_int, C.c_int]),
         "mbg_get_variant_probtraj_squares_raw": (C.c_int, [vp, C.c_int, u64p, u64p]),
         "mbg_get_sweep_info": (C.c_int, [vp, C.POINTER(_SweepInfo)]),
         "mbg_get_sweep_class_info": (C.c_int, [vp, C.c_int, C.POINTER(_SweepClassInfo)]),
@@ -485,18 +486,31 @@ class Context:
         return out.reshape(i.windows, i.readouts)
 
     # ---- variant sweeps ----
+    @staticmethod
+    def _variant_array(vs):
+        arr = (_Variant * len(vs))()
+        for k, (model, n, seed, vid) in enumerate(vs):
+            arr[k] = _Variant(model.handle.value if hasattr(model.handle, "value") else model.handle, n,
+                              0 if seed is None else seed, int(seed is None), vid)
+        return arr
+
     def run_variants(self, variants, error_columns=False):
         """variants: iterable of (Model, n_trajectories, seed or None, variant_id).  All run in one
         HIP graph launch; returns the sweep info (per-variant results through the getters below).
         error_columns: also accumulate the error columns' sums of squares (variant_squares)."""
         vs = list(variants)
-        arr = (_Variant * len(vs))()
+        arr = self._variant_array(vs)
         self._sweep_models = [v[0] for v in vs]  # keep the models alive with the device results
-        for k, (model, n, seed, vid) in enumerate(vs):
-            arr[k] = _Variant(model.handle.value if hasattr(model.handle, "value") else model.handle, n,
-                              0 if seed is None else seed, int(seed is None), vid)
         self._check(lib().mbg_run_variants_ex(self._h, arr, len(vs), 1 if error_columns else 0))
         return self.sweep_info()
+
+    def autotune_variants(self, variants, error_columns=False):
+        """Measures each word class of these variants under candidate layouts and keeps the fastest
+        for later sweeps that form the same classes (mbg_autotune_variants)."""
+        vs = list(variants)
+        arr = self._variant_array(vs)
+        self._sweep_models = [v[0] for v in vs]
+        self._check(lib().mbg_autotune_variants(self._h, arr, len(vs), 1 if error_columns else 0))
 
     def variant_squares(self, v):
         """(prob_sq (W, 2^K), th_sq (W,)) of variant v of a sweep run with error_columns."""

@@ -1425,17 +1425,31 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
 }
 #else
 // Variant sweep: nv models of one state-word class in one persistent launch.  Each wave
-// drains one variant at a time (its descriptor is wave-uniform, read with scalar loads),
-// starting at a variant spread by wave index and moving on when that variant's queue is
-// empty; the variants' tables stay in HBM.
+// drains one variant at a time (its descriptor is wave-uniform, read with scalar loads) and
+// moves on when that variant's queue is empty.
+// Tables in HBM (MBG_TABLES_LDS 0): the waves start at a variant chosen by MBG_SWEEP_XCD.
+// Tables in LDS (MBG_TABLES_LDS 1): every wave of a block works on the block's current variant,
+// whose tables the block stages in the first tab_words words of its LDS (the class's largest
+// table blob); a block starts at variant block_start[block] (a share of the class's work
+// proportional to the variants' work, or variant 0 when block_start is null) and moves on, all
+// its waves together, when that variant's queue is empty.
 // With MBG_TH_LDS each wave sums its current variant's window entropies in its own LDS slice
 // of wmax cells (after the lanes' state) and flushes them when it moves to the next variant.
 // clk (may be null): the class's first block start and last block end on the device's
 // constant-rate clock, by atomic min / max (the class's kernel time inside a graph).
 extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const mbg_desc* descs, int nv, int wmax,
-                                                                           u64* clk) {
+                                                                           u64* clk, int tab_words,
+                                                                           const int* block_start) {
     MBG_DYN_LDS(smem);
     if (clk && threadIdx.x == 0) atomicMin(clk, (u64)wall_clock64());
+#if MBG_TABLES_LDS
+    u64* ctl = smem + tab_words;  // block-uniform flag: the current variant still has work
+    u64* base = ctl + 2;          // the lanes' state (16-byte aligned after the tables)
+    const int start = block_start ? block_start[blockIdx.x] : 0;
+#else
+    (void)tab_words;
+    (void)block_start;
+    u64* base = smem;
 #if MBG_SWEEP_XCD == 2
     // every wave starts at the class's first variant: the host orders them by decreasing work
     // per trajectory, so the costliest trajectories start first and the run ends on the cheapest
@@ -1452,11 +1466,12 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
     const u32 wave = blockIdx.x * (MBG_BLOCK / 64) + (threadIdx.x >> 6);
     const int start = (int)(((u64)wave * (u64)nv) / waves_total);
 #endif
+#endif
 #if MBG_TH_LDS
     // the slice holds the variant's W entropies, then (MBG_ERRORS) their W squares
     const u32 lid = threadIdx.x & 63u;
     const u32 th_cells = (MBG_ERRORS ? 2u : 1u);
-    u64* thw = smem + (MBG_STATE_HBM == 1 ? 0 : MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK +
+    u64* thw = base + (MBG_STATE_HBM == 1 ? 0 : MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK +
                (threadIdx.x >> 6) * th_cells * (u32)wmax;
 #else
     (void)wmax;
@@ -1467,6 +1482,22 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
         if (v >= nv) v -= nv;
         v = __builtin_amdgcn_readfirstlane(v);
         const mbg_desc& D = descs[v];
+#if MBG_TABLES_LDS
+        __syncthreads();  // every wave has left the previous variant: the tables and the flag are free
+        if (threadIdx.x == 0)
+            ctl[0] = __hip_atomic_load(D.work_counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < D.batch ? 1ull : 0ull;
+        __syncthreads();
+        if (ctl[0] == 0ull) continue;  // block-uniform: the variant's queue is already empty
+        {
+            const u32 words = D.lay.bytes / 8;
+            const u64* src = (const u64*)D.blob;
+            for (u32 k = threadIdx.x; k < words; k += MBG_BLOCK) smem[k] = src[k];
+        }
+        __syncthreads();
+        const char* tb = (const char*)smem;
+#else
+        const char* tb = D.blob;
+#endif
 #if MBG_TH_LDS
         for (u32 c = lid; c < th_cells * (u32)D.W; c += 64u) thw[c] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1474,7 +1505,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
 #endif
         // the lanes' state in LDS, or with MBG_STATE_HBM 1 in the class's HBM buffer (every
         // descriptor of the class names the same one)
-        simulate_variant(D, D.blob, MBG_STATE_HBM == 1 ? D.lane_state : smem, thw);
+        simulate_variant(D, tb, MBG_STATE_HBM == 1 ? D.lane_state : base, thw);
 #if MBG_TH_LDS
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();  // every lane of the wave has left the variant's loop

@@ -4,6 +4,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <list>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -269,8 +270,15 @@ struct MargJob {
     int W, K;
 };
 
+// The measured layout decisions of one sweep class (-1: the default rule): tables staged per
+// block in LDS or read from HBM, the active sets in HBM (2) or LDS (0), dependents per round.
+struct ClassChoice {
+    int tables_lds = -1, state_hbm = -1, dep_batch = -1;
+};
+
 struct SweepClass {
     Shape sh{};
+    std::vector<long long> key;  // word class, time mode, error columns, structure of the members
     hipFunction_t sim = nullptr, init_state = nullptr, init_rates = nullptr, init_readouts = nullptr,
                   marginals = nullptr, marginals_batch = nullptr, init_rates_multi = nullptr;
     std::vector<int> members;
@@ -284,6 +292,9 @@ struct SweepClass {
     double kernel_ms = 0;    // the class kernel's span in the last graph (first block start to last end)
     uint64_t trajectories = 0, transitions = 0;
     bool tuned = false;      // the shape came from mbg_autotune_variants
+    uint32_t tab_bytes = 0;  // tables in LDS: the class's largest table blob, staged per block
+    uint64_t grid = 1;       // blocks of the class kernel
+    DevBuf<int> block_start; // tables in LDS: each block's first variant (index into the members)
 };
 
 struct SweepState {
@@ -311,6 +322,8 @@ struct mbg_ctx {
     std::string err;
     std::map<Shape, Module> modules;
     std::map<std::vector<long long>, Shape> tuned;  // measured best shape per model structure
+    std::map<std::vector<long long>, ClassChoice> tuned_classes;  // measured choices per sweep class
+    const ClassChoice* force_class = nullptr;  // the sweep autotuner's candidate (every class)
     RunBuffers buf;
     std::unique_ptr<SweepState> sweep;       // last variant sweep
     std::vector<std::unique_ptr<VariantRun>> variant_pool;  // device buffers reused by later sweeps
@@ -1346,8 +1359,10 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
     // per class: the variants in decreasing order of their expected work per trajectory (the
     // sum of every node's two rates times max_time), the order in which the class kernel's waves
     // drain them: the run ends on the cheapest trajectories, so its tail is short
+    std::vector<double> member_cost(nv, 0.0);
+    std::list<std::vector<int>> keep_starts;  // host sources of the block-start uploads
     for (auto& Cp : sw->classes) {
-        std::vector<double> key(nv, 0.0);
+        std::vector<double>& key = member_cost;
         for (int v : Cp->members) {
             double sum = 0.0;
             for (uint64_t r : rts[v].rates) sum += (double)r;
@@ -1359,9 +1374,35 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
     const size_t cu_lds = 160 * 1024;
     for (auto& Cp : sw->classes) {
         SweepClass& C = *Cp;
+        // tables in LDS: every wave of a block works on the block's current variant, whose tables
+        // the block stages beside the lanes' state (the class's largest blob, plus a 16-byte flag)
+        C.tab_bytes = 0;
+        for (int v : C.members)
+            C.tab_bytes = std::max(C.tab_bytes, pack_blob(sw->vars[v]->m->cm, rts[v], C.sh.rate32 != 0,
+                                                          C.sh.compact != 0).lay.bytes);
+        // the class's key: its shape so far and the structure of its members
+        C.key = {C.sh.nw, C.sh.discrete, C.sh.errors, C.sh.generic, C.sh.prog, C.sh.max_in, (long long)C.members.size()};
+        for (int v : C.members) C.key.push_back(sw->vars[v]->m->cm.N);
+        ClassChoice ch;
+        if (c->force_class) {
+            ch = *c->force_class;
+        } else {
+            const auto tc = c->tuned_classes.find(C.key);
+            if (tc != c->tuned_classes.end()) {
+                ch = tc->second;
+                C.tuned = true;
+            }
+        }
+        // default: the tables in LDS up to 56 KiB (measured on the 256-variant sweep: every class
+        // up to 28 words faster, by 10-30 %; the 29-32-word class, 56-64 KiB of tables, as fast
+        // with them in HBM at more resident waves)
+        const int want_tables = ch.tables_lds >= 0 ? ch.tables_lds
+                                                   : env_int("MBG_TUNE_SWEEP_TABLES", C.tab_bytes <= (56u << 10) ? 1 : 0);
+        C.sh.tables_lds = want_tables && C.tab_bytes <= (64u << 10) ? 1 : 0;
+        const size_t fixed = C.sh.tables_lds ? (size_t)C.tab_bytes + 16 : 0;
         int best_waves = 0;
         for (int block = 1024; block >= 64; block -= 64) {
-            const size_t lds = lane_lds_bytes(C.sh.nw, block, C.sh.wsum_reg);
+            const size_t lds = lane_lds_bytes(C.sh.nw, block, C.sh.wsum_reg) + fixed;
             if (lds > (size_t)c->lds_per_block) continue;
             const int waves = std::min(16, (int)(cu_lds / lds) * (block / 64));
             if (waves > best_waves) {
@@ -1371,21 +1412,22 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
         }
         if (!best_waves) throw Error(MBG_ERR_LIMIT, "model too large: per-lane state does not fit in LDS");
         if (const int bq = env_int("MBG_TUNE_SWEEP_BLOCK", 0)) {
-            if (bq % 64 == 0 && bq <= 1024 && lane_lds_bytes(C.sh.nw, bq, C.sh.wsum_reg) <= (size_t)c->lds_per_block)
+            if (bq % 64 == 0 && bq <= 1024 && lane_lds_bytes(C.sh.nw, bq, C.sh.wsum_reg) + fixed <= (size_t)c->lds_per_block)
                 C.sh.block = bq;
         }
         // the active sets in HBM where that keeps more waves resident (up to 12 per CU, which
         // leave the step loop its VGPRs), as in single runs
         // leave room for the per-wave window-entropy partials
-        if (act_hbm_allowed(C.sh.wsum_reg, C.sh.sumw) && env_int("MBG_TUNE_STATE_HBM", -1) != 0) {
+        if (act_hbm_allowed(C.sh.wsum_reg, C.sh.sumw) &&
+            (ch.state_hbm >= 0 ? ch.state_hbm == 2 : env_int("MBG_TUNE_STATE_HBM", -1) != 0)) {
             int wm = 1;
             for (int v : C.members) {
                 const mbg::CompiledModel& cmv = sw->vars[v]->m->cm;
                 wm = std::max(wm, (int)std::ceil(cmv.max_time / cmv.time_tick));
             }
             int waves2 = 0;
-            const int block2 = act_hbm_block(c, C.sh.nw, 0, (size_t)wm * 8 * (errors ? 2 : 1), &waves2);
-            if (block2 && waves2 > best_waves) {
+            const int block2 = act_hbm_block(c, C.sh.nw, fixed, (size_t)wm * 8 * (errors ? 2 : 1), &waves2);
+            if (block2 && (waves2 > best_waves || ch.state_hbm == 2)) {
                 C.sh.state_hbm = 2;
                 C.sh.block = block2;
             }
@@ -1401,7 +1443,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
             const mbg::CompiledModel& cmv = sw->vars[v]->m->cm;
             wmax = std::max(wmax, (int)std::ceil(cmv.max_time / cmv.time_tick));
         }
-        const size_t state_lds = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg, C.sh.state_hbm);
+        const size_t state_lds = lane_lds_bytes(C.sh.nw, C.sh.block, C.sh.wsum_reg, C.sh.state_hbm) + fixed;
         const size_t th_lds = (size_t)(C.sh.block / 64) * (size_t)wmax * 8 * (errors ? 2 : 1);  // + squares
         // (with the state in HBM: when 16 waves' slices fit in a CU's LDS)
         const bool th_fits = state_lds ? cu_lds / (state_lds + th_lds) == cu_lds / state_lds
@@ -1413,7 +1455,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
         C.wmax = wmax;
         // one dependent per round (measured on the 256-variant sweep: 1.43 s against 1.53 s with
         // two, every class faster or equal)
-        C.sh.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 1)));
+        C.sh.dep_batch = std::max(1, std::min(16, ch.dep_batch > 0 ? ch.dep_batch : env_int("MBG_TUNE_DEP_BATCH", 1)));
         C.sh.dep_cap = 0;
         for (int v : C.members)
             C.sh.dep_cap = std::max(C.sh.dep_cap, default_dep_cap(sw->vars[v]->m->cm, C.sh.dep_batch));
@@ -1438,6 +1480,30 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
             nb = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nb, fit));
         }
         C.blocks_per_cu = nb;
+        {
+            uint64_t traj = 0;
+            for (int v : C.members) traj += sw->vars[v]->n;
+            C.grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * (uint64_t)nb,
+                                                              (traj + C.sh.block - 1) / C.sh.block));
+        }
+        if (C.sh.tables_lds) {
+            // block b starts at the member that holds the point (b + 1/2) / grid of the class's
+            // work (trajectories x expected work per trajectory, members in decreasing order of
+            // the latter), so the blocks on a variant are in proportion to its work
+            std::vector<double> cum;
+            double total = 0.0;
+            for (int v : C.members) {
+                total += (double)sw->vars[v]->n * std::max(member_cost[v], 1e-30);
+                cum.push_back(total);
+            }
+            std::vector<int>& bs = keep_starts.emplace_back(C.grid, 0);
+            for (uint64_t b = 0; b < C.grid; b++) {
+                const double pos = ((double)b + 0.5) / (double)C.grid * total;
+                const size_t k = (size_t)(std::upper_bound(cum.begin(), cum.end(), pos) - cum.begin());
+                bs[b] = (int)std::min(k, cum.size() - 1);
+            }
+            C.block_start.upload(bs, main);
+        }
         if (C.sh.state_hbm)
             C.act.alloc((size_t)c->cus * (size_t)nb * (size_t)C.sh.block * (size_t)C.sh.nw *
                         (C.sh.state_hbm == 2 ? 1 : C.sh.wsum_reg == 1 ? 2 : 3));
@@ -1641,15 +1707,14 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
         SweepClass& C = *sw->classes[ci];
         hipStream_t st = serial ? main : c->class_streams[ci];
         if (!serial) hip_check(hipStreamWaitEvent(st, evs[2 * ncls + 1], 0), "hipStreamWaitEvent");
-        uint64_t traj = 0;
-        for (int v : C.members) traj += sw->vars[v]->n;
-        uint64_t grid = (uint64_t)c->cus * (uint64_t)C.blocks_per_cu;
-        grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (traj + C.sh.block - 1) / C.sh.block));
+        const uint64_t grid = C.grid;
         const mbg_desc_ptr dp = C.descs.p;
         int n_members = (int)C.members.size();
         int wmax_arg = C.wmax;
         uint64_t* clk = SS.clk.p + 2 * ci;
-        void* args[] = {(void*)&dp, &n_members, &wmax_arg, (void*)&clk};
+        int tab_words = (int)(C.sh.tables_lds ? C.tab_bytes / 8 : 0);
+        const int* bstart = C.sh.tables_lds ? C.block_start.p : nullptr;
+        void* args[] = {(void*)&dp, &n_members, &wmax_arg, (void*)&clk, &tab_words, (void*)&bstart};
         hip_check(hipModuleLaunchKernel(C.sim, (unsigned)grid, 1, 1, (unsigned)C.sh.block, 1, 1, (unsigned)C.smem, st,
                                         args, nullptr),
                   "launch mbg_simulate_multi");
@@ -1733,6 +1798,66 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
             c->err_cap *= 4;
             run_variants_impl(c, vs, nv, flags, depth + 1);
         }
+    }
+}
+
+// Sweep autotuner: for each word class of the sweep, the class's variants alone (their full
+// trajectory counts) under candidate layouts, by coordinate descent from the default rule: the
+// tables in LDS or HBM, the active sets in HBM or LDS, one or two dependents per round.  A
+// candidate replaces the best when its class kernel time is more than 1 % shorter.  The
+// choices are kept per class key and applied by later sweeps that form the same class.
+void autotune_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags) {
+    std::map<std::pair<int, int>, std::vector<mbg_variant>> groups;
+    for (int v = 0; v < nv; v++) {
+        if (!vs[v].model) throw Error(MBG_ERR_ARG, "variant without a model");
+        const mbg::CompiledModel& cm = vs[v].model->cm;
+        groups[{word_class(cm.NW), cm.discrete ? 1 : 0}].push_back(vs[v]);
+    }
+    struct Reset {
+        mbg_ctx* c;
+        ~Reset() { c->force_class = nullptr; }
+    } reset{c};
+    for (auto& g : groups) {
+        std::vector<mbg_variant>& sub = g.second;
+        auto time_of = [&](const ClassChoice& cand, std::vector<long long>* key, Shape* shape) {
+            c->force_class = &cand;
+            run_variants_impl(c, sub.data(), (int)sub.size(), flags);
+            c->force_class = nullptr;
+            const SweepClass& C = *c->sweep->classes[0];
+            if (key) *key = C.key;
+            if (shape) *shape = C.sh;
+            return C.kernel_ms > 0 ? C.kernel_ms : (double)c->sweep->info.graph_ms;
+        };
+        ClassChoice best;
+        std::vector<long long> key;
+        Shape sh{};
+        time_of(best, nullptr, nullptr);  // compile and warm the default layout
+        double best_ms = time_of(best, &key, &sh);
+        // resolve the default rule into explicit choices, then flip one decision at a time
+        best.tables_lds = sh.tables_lds;
+        best.state_hbm = sh.state_hbm == 2 ? 2 : 0;
+        best.dep_batch = sh.dep_batch;
+        auto consider = [&](ClassChoice cand) {
+            Shape got{};
+            time_of(cand, nullptr, nullptr);  // compile
+            const double ms = time_of(cand, nullptr, &got);
+            // a flip the layout rules refused (the tables or active sets do not fit) is no candidate
+            if (got.tables_lds != cand.tables_lds || (cand.state_hbm == 2) != (got.state_hbm == 2)) return;
+            if (ms < best_ms * 0.99) {
+                best_ms = ms;
+                best = cand;
+            }
+        };
+        ClassChoice t = best;
+        t.tables_lds = !best.tables_lds;
+        consider(t);
+        t = best;
+        t.state_hbm = best.state_hbm == 2 ? 0 : 2;
+        consider(t);
+        t = best;
+        t.dep_batch = best.dep_batch == 1 ? 2 : 1;
+        consider(t);
+        c->tuned_classes[key] = best;
     }
 }
 
@@ -2346,6 +2471,11 @@ mbg_status mbg_run_variants_ex(mbg_ctx* c, const mbg_variant* variants, int coun
     });
 }
 
+mbg_status mbg_autotune_variants(mbg_ctx* c, const mbg_variant* variants, int count, int flags) {
+    if (!c || !variants || count <= 0) return MBG_ERR_ARG;
+    return guarded(c, [&] { autotune_variants_impl(c, variants, count, flags); });
+}
+
 mbg_status mbg_get_sweep_info(const mbg_ctx* c, mbg_sweep_info* out) {
     if (!c || !out) return MBG_ERR_ARG;
     if (!c->sweep) return MBG_ERR_NOT_BUILT;
@@ -2376,6 +2506,7 @@ mbg_status mbg_get_sweep_class_info(const mbg_ctx* c, int i, mbg_sweep_class_inf
     o.scan = C.sh.scan;
     o.th_lds = C.sh.th_lds;
     o.tuned = C.tuned ? 1 : 0;
+    o.tables_lds = C.sh.tables_lds;
     return MBG_OK;
 }
 
