@@ -24,10 +24,13 @@ launch of the kernel from the FETCH_SIZE and WRITE_SIZE counter summary committe
 profiles/mbg_simulate_large_traffic.txt (each counter scaled by what it reads for a copy kernel
 of known size), null when that file is absent.
 
-Two more objects ride on the same line:
+More objects ride on the same line:
   sweep      256 variant networks of 50-2,000 nodes (rates scaled by log-normal(0, 0.5), 0-3
              knockouts, 1k-64k trajectories each) in one HIP-graph launch: transitions/s and
-             variants/s of the second (warm) sweep.
+             variants/s of the second (warm) sweep, after an untimed per-class autotune
+             (mbg_autotune_variants); the classes' chosen layouts and kernel spans.
+  fixed_heavy the 5,000-node fixed-point-heavy network (262,144 trajectories, max_time 200):
+             transitions/s per step and in the kernel, with its own VALU roofline figure.
   surrogate  the sweep surrogate on the 200-node network (50,000 simulated training variants,
              4-member BF16 MLP ensemble, 10,000 held-out variants, spread gate 0.01, judged
              against 262,144-trajectory simulations): accepted fraction, errors, points/s of
@@ -150,6 +153,50 @@ def surrogate_bench():
     }
 
 
+def fixed_heavy_bench(ctx, steps=2, tune=True):
+    """The fixed-point-heavy workload: 5,000 nodes (79 state words), in-degree 1-3, 30 % of the
+    nodes self-sustaining (x = x | f) or absorbing (rate_down 0), 262,144 trajectories, max_time
+    200, time_tick 2.  Most trajectories end at a fixed point; every fixed point is a distinct
+    5,000-bit state, deduplicated on the device.  Same step definition and roofline as the
+    headline line (W(N, d, k) lane-ops per transition against the VALU peak)."""
+    from upmabossg_amd import Model, synth
+
+    net = synth.config_fixed_heavy()
+    m = Model(net.bnd(), net.cfg())
+    ins = net.inputs()
+    mean_in = sum(len(i) for i in ins) / net.n
+    ops = algorithmic_ops(net.n, mean_in, mean_in)
+    tune_info = ctx.autotune(m) if tune else None
+    ctx.run(m, fetch=False)  # compile, buffers
+    walls, kms, trans = [], [], 0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        ctx.run(m, fetch=False)
+        walls.append(time.perf_counter() - t0)
+        i = ctx.info()
+        kms.append(i.kernel_ms)
+        trans += i.transitions
+    i = ctx.info()
+    kern_s = sum(kms) / len(kms) / 1e3
+    per_launch = trans / steps
+    achieved = per_launch * ops / kern_s / 1e12
+    return {
+        "workload": f"fixed-point heavy: {net.n}-node network, in-degree 1-3, 30 % self-sustaining or absorbing "
+                    f"nodes, {i.simulated} trajectories, max_time 200, time_tick 2",
+        "trajectories": int(i.simulated),
+        "transitions_per_step": per_launch,
+        "fixed_point_trajectories": int(i.fixed_point_trajectories),
+        "ms_per_step": sum(walls) / len(walls) * 1e3,
+        "kernel_ms": kern_s * 1e3,
+        "transitions_per_s": trans / sum(walls),
+        "transitions_per_s_kernel": per_launch / kern_s,
+        "launch_shape": ({k: tune_info[k] for k in ("block", "tables_in_lds", "wsum_reg", "state_hbm", "dep_batch",
+                                                     "dep_cap")} if tune_info else "untuned"),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "T lane-op/s",
+                     "frac": achieved / VALU_PEAK_TOPS, "ops_per_transition": ops},
+    }
+
+
 def ring_bench(ctx, sizes=(100, 500), cpu=True):
     """The ring models of the reference's own generator (data/generate-synth.py defaults: 5
     readouts, 1,000,000 trajectories, time_tick 5, max_time 100, signal length int(sqrt(N)),
@@ -225,6 +272,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-ring", action="store_true", help="skip the reference generator's ring models")
+    ap.add_argument("--no-fixed-heavy", action="store_true", help="skip the 5,000-node fixed-point-heavy workload")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the untuned launch shape (and the MBG_TUNE_* environment knobs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -276,6 +324,13 @@ def main():
             except Exception as e:  # reported, never fatal to the headline line
                 sweep = {"error": f"{type(e).__name__}: {e}"}
             print(f"sweep: {sweep}", file=sys.stderr, flush=True)
+        fixed_heavy = None
+        if not args.no_fixed_heavy:
+            try:
+                fixed_heavy = fixed_heavy_bench(ctx, tune=not args.no_autotune)
+            except Exception as e:  # reported, never fatal to the headline line
+                fixed_heavy = {"error": f"{type(e).__name__}: {e}"}
+            print(f"fixed_heavy: {fixed_heavy}", file=sys.stderr, flush=True)
         ring = None
         if not args.no_ring:
             try:
@@ -335,6 +390,8 @@ def main():
     }
     if sweep is not None:
         out["sweep"] = sweep
+    if fixed_heavy is not None:
+        out["fixed_heavy"] = fixed_heavy
     if ring is not None:
         out["reference_generator_rings"] = ring
     if not args.no_surrogate:

@@ -171,14 +171,30 @@ __global__ void gelu_backward(const float* __restrict__ dH, const float* __restr
     dZb[i] = (bf16)v;
 }
 
-// out[c] = sum over rows of X[r][c] (bias gradients, loss sums).
-__global__ void column_sums(const float* __restrict__ X, float* __restrict__ out, long rows, int cols) {
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= cols) return;
+// out[c] = sum over rows of X[r][c] (bias gradients, loss sums).  A block of 1,024 threads
+// takes 64 columns: its 16 waves each sum every 16th row of them (a wave reads 64 consecutive
+// floats per row), then wave 0 adds the 16 partial sums from LDS in a fixed order, so the
+// result does not depend on scheduling.
+constexpr int kColTile = 64, kRowGroups = 16;
+__global__ __launch_bounds__(1024) void column_sums(const float* __restrict__ X, float* __restrict__ out, long rows,
+                                                    int cols) {
+    __shared__ float part[kRowGroups][kColTile];
+    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int c = blockIdx.x * kColTile + lane;
     float s = 0.f;
-    for (long r = 0; r < rows; r++) s += X[r * cols + c];
-    out[c] = s;
+    if (c < cols) {
+#pragma unroll 4
+        for (long r = grp; r < rows; r += kRowGroups) s += X[r * cols + c];
+    }
+    part[grp][lane] = s;
+    __syncthreads();
+    if (grp == 0 && c < cols) {
+        float t = 0.f;
+        for (int k = 0; k < kRowGroups; k++) t += part[k][lane];
+        out[c] = t;
+    }
 }
+unsigned col_blocks(int cols) { return (unsigned)((cols + kColTile - 1) / kColTile); }
 
 // Gather the rows idx[r] of the device training set into a batch (BF16 inputs, FP32 targets).
 __global__ void gather_rows(const bf16* __restrict__ Xs, const float* __restrict__ Ys, const uint32_t* __restrict__ idx,
@@ -372,7 +388,7 @@ struct mbg_mlp {
         for (int l = L - 1; l >= 0; l--) {
             // weight gradient dW[out][in] = dZ^T H_l ; bias gradient = column sums of dZ
             gemm(dZb.p, dout[l], false, H[l].p, din[l], false, g.p + woff[l], din[l], dout[l], din[l], rows, false);
-            column_sums<<<blocks(dout[l]), 256, 0, s>>>(dZ.p, g.p + boff[l], rows, dout[l]);
+            column_sums<<<col_blocks(dout[l]), 1024, 0, s>>>(dZ.p, g.p + boff[l], rows, dout[l]);
             if (l > 0) {
                 // dH = dZ W (rows x din) ; dZ_{l-1} = dH * GELU'(Z_{l-1})
                 gemm(dZb.p, dout[l], true, wb.p + woff[l], din[l], false, dH.p, din[l], rows, din[l], dout[l], false);
@@ -382,7 +398,7 @@ struct mbg_mlp {
     }
 
     float loss(long rows) {
-        column_sums<<<blocks(n_out), 256, 0, s>>>(sq.p, sums.p, rows, n_out);
+        column_sums<<<col_blocks(n_out), 1024, 0, s>>>(sq.p, sums.p, rows, n_out);
         std::vector<float> h(n_out);
         check(hipMemcpyAsync(h.data(), sums.p, n_out * 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
         check(hipStreamSynchronize(s), "loss");

@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import json
 import math
+import os
 import sys
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -90,7 +91,10 @@ class Ensemble:
             return loss
 
         # members are independent (own buffers and stream): train them concurrently so their
-        # small per-step kernels overlap on the device
+        # small per-step kernels overlap on the device (MBG_SURROGATE_SERIAL=1: one after
+        # another, for tools that trace kernel launches from a single host thread)
+        if os.environ.get("MBG_SURROGATE_SERIAL", "0") == "1":
+            return [one(k) for k in range(len(self.members))]
         with ThreadPoolExecutor(max_workers=len(self.members)) as pool:
             return list(pool.map(one, range(len(self.members))))
 
