@@ -1223,18 +1223,16 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         u64 acc = 0, sgs = 1;
         int gsel = NSUM - 1;
         bool found = false;
+        // branch-free: selects on a lane mask instead of a branch per group
 #pragma unroll
         for (int g = 0; g < NSUM; g++) {
             const u64 sg = LANE_S(g);
-            if (!found) {
-                if (acc + sg > target) {
-                    gsel = g;
-                    sgs = sg;
-                    found = true;
-                } else {
-                    acc += sg;
-                }
-            }
+            const u64 na = acc + sg;
+            const bool hit = !found && na > target;
+            gsel = hit ? g : gsel;
+            sgs = hit ? sg : sgs;
+            acc = (found || hit) ? acc : na;
+            found = found || hit;
         }
         // In the group, the selected node is the first (in node order) whose inclusive prefix
         // of rates exceeds rem = target - acc.  Equivalently it is the first from the top whose

@@ -324,6 +324,9 @@ struct mbg_ctx {
     std::map<std::vector<long long>, Shape> tuned;  // measured best shape per model structure
     std::map<std::vector<long long>, ClassChoice> tuned_classes;  // measured choices per sweep class
     const ClassChoice* force_class = nullptr;  // the sweep autotuner's candidate (every class)
+    // transitions per trajectory measured by earlier sweeps, per variant (model, trajectories,
+    // seed, variant id): the work estimate that orders a class's variants and shares its blocks
+    std::map<std::vector<uint64_t>, double> variant_cost;
     RunBuffers buf;
     std::unique_ptr<SweepState> sweep;       // last variant sweep
     std::vector<std::unique_ptr<VariantRun>> variant_pool;  // device buffers reused by later sweeps
@@ -1291,6 +1294,11 @@ int word_class(int nw) {
     return c;
 }
 
+std::vector<uint64_t> variant_key(const mbg_variant& v) {
+    return {(uint64_t)(uintptr_t)v.model, v.n_trajectories, v.use_model_seed ? ~0ull : v.seed, v.variant_id,
+            v.model ? (uint64_t)v.model->cm.N : 0ull};
+}
+
 // Variant sweep: all variants in one HIP graph launch.
 void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int depth = 0) {
     const bool errors = (flags & MBG_SWEEP_ERROR_COLUMNS) != 0;
@@ -1356,18 +1364,37 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
         C.sh.wsmad &= rates_fit_i31(rts[v]) ? 1 : 0;
         sw->vars.push_back(std::move(V));
     }
-    // per class: the variants in decreasing order of their expected work per trajectory (the
-    // sum of every node's two rates times max_time), the order in which the class kernel's waves
-    // drain them: the run ends on the cheapest trajectories, so its tail is short
+    // per class: the variants in decreasing order of their expected work per trajectory, the
+    // order in which the class kernel's waves drain them (the run ends on the cheapest
+    // trajectories, so its tail is short) and the share of blocks each gets with the tables in
+    // LDS.  The expectation is the transitions per trajectory an earlier sweep of the variant
+    // measured, else a rate estimate: the mean of every node's two rates times max_time (which
+    // misses the active fraction: 30-45 % off from variant to variant on random networks)
     std::vector<double> member_cost(nv, 0.0);
     std::list<std::vector<int>> keep_starts;  // host sources of the block-start uploads
     for (auto& Cp : sw->classes) {
         std::vector<double>& key = member_cost;
+        double ratio = 0.0;  // measured / estimated, summed over the members measured before
+        int measured = 0;
+        std::vector<char> known(nv, 0);
         for (int v : Cp->members) {
             double sum = 0.0;
             for (uint64_t r : rts[v].rates) sum += (double)r;
-            key[v] = std::ldexp(sum, -rts[v].E) * (double)sw->vars[v]->m->cm.max_time;
+            const double est = std::ldexp(sum, -rts[v].E) * (double)sw->vars[v]->m->cm.max_time;
+            const auto mc = c->variant_cost.find(variant_key(vs[v]));
+            if (mc != c->variant_cost.end() && est > 0) {
+                key[v] = mc->second;
+                ratio += mc->second / est;
+                measured++;
+                known[v] = 1;
+            } else {
+                key[v] = est;
+            }
         }
+        // estimates on the scale of the measured members (when some are)
+        if (measured)
+            for (int v : Cp->members)
+                if (!known[v]) key[v] *= ratio / measured;
         std::stable_sort(Cp->members.begin(), Cp->members.end(), [&](int a, int b) { return key[a] > key[b]; });
     }
     // per class: block size for the lanes' LDS, module, occupancy
@@ -1777,6 +1804,7 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
         if (nfp > V.D.fp_cap) throw Error(MBG_ERR_LIMIT, "fixed-point buffer overflow");
         collect_fixed_points(c, *V.mod, V.buf.fp.p, nfp, V.NWc, V.m->cm.NW, main, V.fixed_states, V.fixed_counts);
         V.info.transitions = flips;
+        c->variant_cost[variant_key(vs[v])] = (double)flips / (double)V.n;
         V.info.fixed_point_trajectories = nfp;
         V.info.kernel_ms = ms;
         SI.trajectories += V.n;
