@@ -215,8 +215,8 @@ mbg_status mbg_run_variants(mbg_ctx* c, const mbg_variant* variants, int count);
 mbg_status mbg_run_variants_ex(mbg_ctx* c, const mbg_variant* variants, int count, int flags);
 /* Sweep autotuner (no reference counterpart: the reference has no sweeps).  For each word class
  * that these variants form, runs the class's variants alone under candidate layouts (tables
- * staged per block in LDS or read from HBM, active sets in HBM or LDS, one or two dependents
- * per round), keeps the fastest per class, and applies it to later sweeps that form the same
+ * staged per block in LDS or read from HBM, active sets in HBM or LDS, dependents re-evaluated
+ * per loop iteration), keeps the fastest per class, and applies it to later sweeps that form the same
  * class (mbg_sweep_class_info.tuned).  Results never depend on the layout.  flags as for
  * mbg_run_variants_ex.  Leaves the last tuning run as the context's last sweep. */
 mbg_status mbg_autotune_variants(mbg_ctx* c, const mbg_variant* variants, int count, int flags);

@@ -137,6 +137,19 @@ def test_fixed_points_deduplicated_on_device_bit_exact(gpu_ctx, n):
     assert g.fixed_point_trajectories > 4096 and len(g.fixed_counts) == len(r.fixed_counts)
 
 
+def test_dependent_batches_refused(gpu_ctx, monkeypatch):
+    """More than one dependent per round of table reads gave wrong, varying transition counts
+    on the GPU in several launch shapes (the kernel source is exact in the CPU emulation); the
+    runtime refuses such a shape with a clear error instead of running it."""
+    from upmabossg_amd import MbgError
+
+    monkeypatch.setenv("MBG_TUNE_DEP_BATCH", "2")
+    net = synth.config_small()
+    with pytest.raises(MbgError) as e:
+        gpu_ctx.run(Model(net.bnd(), net.cfg()), count=100)
+    assert e.value.status == 2 and "one dependent per round" in str(e.value)
+
+
 @pytest.mark.parametrize("cap", ["1", "2", "0"])
 def test_dependents_cap_bit_exact(gpu_ctx, monkeypatch, cap):
     """A cap on the dependents re-evaluated per loop iteration only reschedules the work (a lane
@@ -144,7 +157,6 @@ def test_dependents_cap_bit_exact(gpu_ctx, monkeypatch, cap):
     deferred (1), two per iteration, or no cap, the results are the oracle's.  The cap is read
     when a launch shape is built, so each value compiles its own kernel."""
     monkeypatch.setenv("MBG_TUNE_DEP_CAP", cap)
-    monkeypatch.setenv("MBG_TUNE_DEP_BATCH", "1")
     net = synth.random_network(300, 1, 6, 5, rates="loguniform", istate="uniform", readouts=12, special_frac=0.2,
                                max_time=20.0, time_tick=0.5, sample_count=3000)
     _compare(gpu_ctx, net.bnd(), net.cfg(), count=3000)
@@ -179,15 +191,12 @@ def test_lane_state_in_hbm_bit_exact(gpu_ctx, monkeypatch, case):
         _compare(gpu_ctx, net.bnd(), net.cfg(), count=512, n=262144)
 
 
-@pytest.mark.parametrize("case,batch", [("random", "1"), ("random", "3"), ("sparse", "2"), ("ring", "1"),
-                                        ("discrete", "2"), ("grid", "1")])
-def test_active_sets_in_hbm_bit_exact(gpu_ctx, monkeypatch, case, batch):
+@pytest.mark.parametrize("case", ["random", "sparse", "ring", "discrete", "grid"])
+def test_active_sets_in_hbm_bit_exact(gpu_ctx, monkeypatch, case):
     """Only the active sets in HBM (x in LDS, word sums in VGPRs: MBG_STATE_HBM 2, half the LDS
-    per lane) is a layout only.  With one dependent per round and with batches of 2-3, whose
-    act words may coincide, on random, sparse-table, logic-program and discrete models and on a
-    3-block grid, the results are the oracle's."""
+    per lane) is a layout only: on random, sparse-table, logic-program and discrete models and on
+    a 3-block grid, the results are the oracle's."""
     monkeypatch.setenv("MBG_TUNE_STATE_HBM", "2")
-    monkeypatch.setenv("MBG_TUNE_DEP_BATCH", batch)
     if case in ("random", "grid"):
         net = synth.random_network(300, 1, 6, 7, rates="loguniform", istate="uniform", readouts=12,
                                    special_frac=0.2, max_time=20.0, time_tick=0.5, sample_count=3000)

@@ -161,7 +161,7 @@ struct Desc {
 using mbg_desc_ptr = const void*;
 
 struct Shape {
-    int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 2, scan = 4, wsum_reg = 0, rate32 = 0,
+    int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 1, scan = 4, wsum_reg = 0, rate32 = 0,
         multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0, th_lds = 0, cells_lds = 0, dep_cap = 0,
         state_hbm = 0, errors = 0, sweep_xcd = 0;
     bool operator<(const Shape& o) const {
@@ -271,9 +271,10 @@ struct MargJob {
 };
 
 // The measured layout decisions of one sweep class (-1: the default rule): tables staged per
-// block in LDS or read from HBM, the active sets in HBM (2) or LDS (0), dependents per round.
+// block in LDS or read from HBM, the active sets in HBM (2) or LDS (0), dependents re-evaluated
+// per loop iteration (0: no cap).
 struct ClassChoice {
-    int tables_lds = -1, state_hbm = -1, dep_batch = -1;
+    int tables_lds = -1, state_hbm = -1, dep_cap = -1;
 };
 
 struct SweepClass {
@@ -458,6 +459,20 @@ int default_sumw(int nw, int wsum_reg) {
 // splits each flip's work over more iterations.  Default: 1.5 times the mean dependent count,
 // at least 4 and at least two batches; MBG_TUNE_DEP_CAP sets it (0 removes the cap).  The last
 // round of a capped iteration may be a partial batch.
+// Dependents re-evaluated per round of independent table reads: always 1.  With 2 or more
+// (MBG_DEP_BATCH > 1) the GPU gave wrong transition counts that changed from run to run in
+// several launch shapes (with kernels compiled at -O3: a 177-node model in blocks of 512 and
+// 1,024, 4-, 20-, 24- and 28-word sweep classes, 60- and 130-node models; at -O1: a 4-word sweep
+// class with its tables per block in LDS and blocks of 1,024), while the same kernel source runs
+// bit-exact in the CPU emulation of tests/kernel_emu for those shapes and batches.  Every case
+// that went wrong had batches of 2; with 1 every case is exact.  The cause is not identified, so
+// the layout is refused (MBG_TUNE_DEP_BATCH may only ask for 1).
+int dependents_per_round() {
+    const int v = env_int("MBG_TUNE_DEP_BATCH", 1);
+    if (v != 1) throw Error(MBG_ERR_LIMIT, "MBG_TUNE_DEP_BATCH: only one dependent per round is supported");
+    return 1;
+}
+
 int default_dep_cap(const mbg::CompiledModel& cm, int dep_batch) {
     double sum = 0;
     for (const mbg::NodeRec& n : cm.nodes) sum += n.dep_cnt;
@@ -497,7 +512,7 @@ Shape choose_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, uint32_t blob
             }
         }
     }
-    best.dep_batch = std::max(1, std::min(16, env_int("MBG_TUNE_DEP_BATCH", 2)));
+    best.dep_batch = dependents_per_round();
     best.dep_cap = default_dep_cap(cm, best.dep_batch);
     best.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
     best.sumw = default_sumw(cm.NW, best.wsum_reg);
@@ -1237,12 +1252,8 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
     // step knobs on the best layout
     {
         const Shape lay = best;
-        for (int db : {1, 3, 4}) {
-            Shape sh = lay;
-            sh.dep_batch = db;
-            consider(sh);
-        }
-        // dependents per iteration: no cap, or a few rounds of the chosen batch
+        // dependents per iteration: no cap, or a few rounds (one dependent per round, see
+        // dependents_per_round)
         {
             const Shape b = best;
             for (int rounds : {0, 2, 3, 4, 6}) {
@@ -1480,12 +1491,11 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
                           ? 1
                           : 0;
         C.wmax = wmax;
-        // one dependent per round (measured on the 256-variant sweep: 1.43 s against 1.53 s with
-        // two, every class faster or equal)
-        C.sh.dep_batch = std::max(1, std::min(16, ch.dep_batch > 0 ? ch.dep_batch : env_int("MBG_TUNE_DEP_BATCH", 1)));
+        C.sh.dep_batch = dependents_per_round();
         C.sh.dep_cap = 0;
         for (int v : C.members)
             C.sh.dep_cap = std::max(C.sh.dep_cap, default_dep_cap(sw->vars[v]->m->cm, C.sh.dep_batch));
+        if (ch.dep_cap >= 0) C.sh.dep_cap = ch.dep_cap;
         C.sh.scan = env_int("MBG_TUNE_SCAN", 4) >= 8 ? 8 : 4;
         const Module& mod = get_module(c, sw->vars[C.members[0]]->m->cm, C.sh);
         for (int v : C.members) sw->vars[v]->mod = &mod;
@@ -1831,7 +1841,8 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
 
 // Sweep autotuner: for each word class of the sweep, the class's variants alone (their full
 // trajectory counts) under candidate layouts, by coordinate descent from the default rule: the
-// tables in LDS or HBM, the active sets in HBM or LDS, one or two dependents per round.  A
+// tables in LDS or HBM, the active sets in HBM or LDS, the dependents cap (none, 3 or the
+// default).  A
 // candidate replaces the best when its class kernel time is more than 1 % shorter.  The
 // choices are kept per class key and applied by later sweeps that form the same class.
 void autotune_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags) {
@@ -1864,7 +1875,7 @@ void autotune_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags
         // resolve the default rule into explicit choices, then flip one decision at a time
         best.tables_lds = sh.tables_lds;
         best.state_hbm = sh.state_hbm == 2 ? 2 : 0;
-        best.dep_batch = sh.dep_batch;
+        best.dep_cap = sh.dep_cap;
         auto consider = [&](ClassChoice cand) {
             Shape got{};
             time_of(cand, nullptr, nullptr);  // compile
@@ -1882,9 +1893,12 @@ void autotune_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags
         t = best;
         t.state_hbm = best.state_hbm == 2 ? 0 : 2;
         consider(t);
-        t = best;
-        t.dep_batch = best.dep_batch == 1 ? 2 : 1;
-        consider(t);
+        for (int cap : {0, 3}) {
+            if (cap == best.dep_cap) continue;
+            t = best;
+            t.dep_cap = cap;
+            consider(t);
+        }
         c->tuned_classes[key] = best;
     }
 }
