@@ -644,8 +644,8 @@ extern "C" __global__ __launch_bounds__(256) void mbg_init_rates(mbg_desc D) {
 #if MBG_MULTI
 // The same pass for every variant of a sweep class in one launch: block (x, v) takes
 // trajectories 64x .. 64x + 63 of variant v.
-extern "C" __global__ __launch_bounds__(256) void mbg_init_rates_multi(const mbg_desc* descs) {
-    const mbg_desc& D = descs[blockIdx.y];
+extern "C" __global__ __launch_bounds__(256) void mbg_init_rates_multi(const CONST_AS mbg_desc* descs) {
+    const mbg_desc& D = *(const mbg_desc*)(descs + blockIdx.y);
     const u64 t0 = (u64)blockIdx.x * INIT_TRAJ;
     if (t0 >= D.batch) return;  // the whole block: it holds no barrier yet
     init_rates_block(D, t0);
@@ -1435,7 +1435,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
 // of wmax cells (after the lanes' state) and flushes them when it moves to the next variant.
 // clk (may be null): the class's first block start and last block end on the device's
 // constant-rate clock, by atomic min / max (the class's kernel time inside a graph).
-extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const mbg_desc* descs, int nv, int wmax,
+extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const CONST_AS mbg_desc* descs, int nv, int wmax,
                                                                            u64* clk, int tab_words,
                                                                            const int* block_start) {
     MBG_DYN_LDS(smem);
@@ -1479,7 +1479,10 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate_multi(const
         int v = start + i;
         if (v >= nv) v -= nv;
         v = __builtin_amdgcn_readfirstlane(v);
-        const mbg_desc& D = descs[v];
+        // the descriptors are constant during the launch: read through the constant address
+        // space, their fields are scalar loads the compiler may repeat where it needs them,
+        // instead of vector loads kept in VGPRs or reloaded and waited for inside the step loop
+        const mbg_desc& D = *(const mbg_desc*)(descs + v);
 #if MBG_TABLES_LDS
         __syncthreads();  // every wave has left the previous variant: the tables and the flag are free
         if (threadIdx.x == 0)
