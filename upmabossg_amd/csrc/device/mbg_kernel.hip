@@ -711,14 +711,23 @@ struct Lane {
     __device__ __forceinline__ u32 xbit(u32 j) const { return (u32)(LANE_X(j >> 6) >> (j & 63)) & 1u; }
 
     __device__ __forceinline__ u32 key_of(const mbg_nd& n, u32 i) const {
-        u32 key = xbit(i);
 #if MBG_MAX_IN <= 6
         // branch-free: reads past a node's inputs hit the next node's entries (or the
-        // table's padding); their bits are masked off once at the end
+        // table's padding); their bits are masked off once at the end.  Every input id, then
+        // every state word, is read before any is used, so the LDS reads overlap
+        u32 id[MBG_MAX_IN];
+        u64 xw[MBG_MAX_IN];
 #pragma unroll
-        for (u32 m = 0; m < MBG_MAX_IN; m++) key |= xbit(TIN(n.in_off + m)) << (m + 1);
+        for (u32 m = 0; m < MBG_MAX_IN; m++) id[m] = TIN(n.in_off + m);
+        const u64 own = LANE_X(i >> 6);
+#pragma unroll
+        for (u32 m = 0; m < MBG_MAX_IN; m++) xw[m] = LANE_X(id[m] >> 6);
+        u32 key = (u32)(own >> (i & 63)) & 1u;
+#pragma unroll
+        for (u32 m = 0; m < MBG_MAX_IN; m++) key |= ((u32)(xw[m] >> (id[m] & 63)) & 1u) << (m + 1);
         key &= (2u << IN_CNT(n)) - 1u;
 #else
+        u32 key = xbit(i);
         for (u32 m = 0; m < IN_CNT(n); m++) key |= xbit(TIN(n.in_off + m)) << (m + 1);
 #endif
         return key;
