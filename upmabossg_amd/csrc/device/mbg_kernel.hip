@@ -961,27 +961,39 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     have = true;
                     hold = odd_iter;
                     id = D.batch_first + g;
-                    const u64* st = D.stage + g * STAGE_ROW;
+                    // the staged rows are not written while this kernel runs: read through the
+                    // constant address space, their loads need not wait for the lane-state
+                    // stores in between (which the compiler cannot otherwise tell apart)
+                    const CONST_AS u64* st = (const CONST_AS u64*)(D.stage + g * STAGE_ROW);
                     total = 0;
-#pragma unroll
-                    for (int g = 0; g < NSUM; g++) {
-                        u64 gs = 0;
-                        // groups of several words are copied word by word, so that the copies
-                        // of a wide state do not all sit in registers at once
+                    // wide states one group at a time, so that the copies of a whole state do
+                    // not all sit in registers at once; a group's words are all read before
+                    // any is written, so the reads overlap
 #if MBG_SUMW > 2
 #pragma unroll 1
 #else
 #pragma unroll
 #endif
+                    for (int g = 0; g < NSUM; g++) {
+                        u64 vx[MBG_SUMW], va[MBG_SUMW], vw[MBG_SUMW];
+#pragma unroll
                         for (int q = 0; q < MBG_SUMW; q++) {
                             const int w = g * MBG_SUMW + q;
                             if (MBG_WSUM_REG == 2 && w >= MBG_NW) continue;  // short last group
-                            LANE_X(w) = st[w];
-                            LANE_A(w) = st[MBG_NW + w];
-
-                            gs += st[2 * MBG_NW + w];
+                            vx[q] = st[w];
+                            va[q] = st[MBG_NW + w];
+                            vw[q] = st[2 * MBG_NW + w];
+                        }
+                        u64 gs = 0;
+#pragma unroll
+                        for (int q = 0; q < MBG_SUMW; q++) {
+                            const int w = g * MBG_SUMW + q;
+                            if (MBG_WSUM_REG == 2 && w >= MBG_NW) continue;
+                            LANE_X(w) = vx[q];
+                            LANE_A(w) = va[q];
+                            gs += vw[q];
 #if MBG_WSUM_REG == 2
-                            LANE_W(w) = st[2 * MBG_NW + w];
+                            LANE_W(w) = vw[q];
 #endif
                         }
                         LANE_S(g) = gs;
