@@ -125,7 +125,7 @@ int main(int argc, char** argv) {
         const uint64_t stride = (uint64_t)grid * MBG_BLOCK + stride_extra;
         std::vector<u64> lane_state;
 #if MBG_STATE_HBM
-        lane_state.assign((size_t)stride * (MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW,
+        lane_state.assign((size_t)stride * (MBG_STATE_HBM == 2 ? 1 : MBG_STATE_HBM == 3 || MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW,
                           0x5A5A5A5A5A5A5A5Aull);
         emu_poison(lane_state.data(), lane_state.size() * 8);
 #endif
@@ -256,7 +256,7 @@ int main(int argc, char** argv) {
         emu_unpoison(&grid, sizeof(grid));
         const size_t th_cells = MBG_ERRORS ? 2 : 1;  // window entropies, then their squares
         const size_t lane_words =
-            MBG_STATE_HBM == 1 ? 0 : (size_t)(MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
+            MBG_STATE_HBM == 1 ? 0 : (size_t)(MBG_STATE_HBM >= 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
 #if !MBG_MULTI
         Variant& V = *vars[0];
         mbg_desc& D = V.D;

@@ -222,6 +222,42 @@ def test_active_sets_in_hbm_bit_exact(gpu_ctx, monkeypatch, case):
     assert gpu_ctx.info().state_hbm == 2
 
 
+@pytest.mark.parametrize("case", ["random", "sparse", "ring", "discrete", "grid", "wide"])
+def test_x_in_lds_active_sets_and_sums_in_hbm_bit_exact(gpu_ctx, monkeypatch, case):
+    """x alone in LDS, the active sets and word sums in HBM, group sums in VGPRs (MBG_STATE_HBM
+    3, two-level sums) is a layout only: on random, sparse-table, logic-program and discrete
+    models, on a 3-block grid and on the 79-word fixed-point-heavy network, the results are the
+    oracle's."""
+    monkeypatch.setenv("MBG_TUNE_STATE_HBM", "3")
+    monkeypatch.setenv("MBG_TUNE_WSUM_REG", "2")
+    if case in ("random", "grid"):
+        net = synth.random_network(300, 1, 6, 7, rates="loguniform", istate="uniform", readouts=12,
+                                   special_frac=0.2, max_time=20.0, time_tick=0.5, sample_count=3000)
+        if case == "grid":
+            m = Model(net.bnd(), net.cfg())
+            a = gpu_ctx.run(m, count=3000, trace=True, block=128, grid=3)
+            assert gpu_ctx.info().state_hbm == 3
+            r = OracleModel(net.bnd(), net.cfg()).run(count=3000, threads=8, record_summaries=True)
+            assert np.array_equal(a.summaries[:, 1], r.summaries["flip_hash"])
+            assert np.array_equal(a.prob_raw, r.prob)
+            return
+        _compare(gpu_ctx, net.bnd(), net.cfg(), count=3000)
+    elif case == "sparse":
+        net = synth.random_network(120, 2, 3, 9, rates="loguniform", istate="uniform", readouts=30,
+                                   max_time=10.0, time_tick=0.5, sample_count=2000)
+        _compare(gpu_ctx, net.bnd(), net.cfg(), count=2000)
+    elif case == "ring":
+        b, c = synth.ring_network(100, steps_to_fixed_point=50, sample_count=2000, time_tick=5.0, max_time=100.0)
+        _compare(gpu_ctx, b, c, count=2000)
+    elif case == "discrete":
+        b, c = model_text("cellcycle")
+        _compare(gpu_ctx, b, c, count=5000)
+    else:
+        net = synth.config_fixed_heavy()
+        _compare(gpu_ctx, net.bnd(), net.cfg(), count=512, n=262144)
+    assert gpu_ctx.info().state_hbm == 3
+
+
 def test_more_than_128_words_bit_exact(gpu_ctx):
     """8,300 nodes (130 words): the initial states are drawn per (trajectory, word), the readout
     part of each staged state by its own pass, and the lanes' state (3,120 bytes per lane, more

@@ -103,8 +103,8 @@ def check_kernel(verbose=False):
             th = 1  # window-entropy partials in LDS (per block, or per wave in sweeps)
             cells = 1 if (th and not sp and nw <= 2) else 0  # small models: the whole window table too
             cap = 2 * db if i % 2 else 0  # dependents per iteration capped in every other shape
-            # lanes' state in HBM (1), or only their active sets (2)
-            hbm = 1 if (not multi and i in (3, 5)) else 2 if i in (1, 4, 8) else 0
+            # lanes' state in HBM (1), only their active sets (2), or all but x (3)
+            hbm = 1 if (not multi and i in (3, 5)) else 2 if i in (1, 4, 8) else 3 if i == 6 else 0
             errors = 1 if i in (1, 2, 4) else 0  # the error columns' sums: dense, sweep and sparse
             with open(src, "w") as f:
                 f.write(kernel_source(nw, block, disc, trace, gen, lds, mi, db, sc, wr, r32, multi, sumw, cp, mad, sp,

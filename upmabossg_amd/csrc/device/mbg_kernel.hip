@@ -33,7 +33,10 @@
 //                   a CU with one or two waves (single runs only); 2: only act in that buffer,
 //                   also in sweeps (a class's grid shares one buffer, lane by lane),
 //                   x in LDS and the word sums in VGPRs (half the LDS per lane, so twice the
-//                   resident waves; act is read once per step and once per rate change)
+//                   resident waves; act is read once per step and once per rate change);
+//                   3: act and the word sums in that buffer, x in LDS and the group sums in
+//                   VGPRs (MBG_WSUM_REG 2; wide models, single runs: the input gathers of the
+//                   dependents and the flips stay in LDS)
 // and mbg_math.h.
 //
 // One trajectory per lane (the reference runs one CUDA thread per trajectory,
@@ -149,6 +152,9 @@ typedef unsigned int sidx;
 // The active sets alone in HBM keep x alone in LDS: the rate sums must be in registers.
 #if MBG_STATE_HBM == 2 && MBG_WSUM_REG != 1
 #error "the active sets alone in HBM need register rate sums"
+#endif
+#if MBG_STATE_HBM == 3 && (MBG_WSUM_REG != 2 || MBG_MULTI)
+#error "the active sets and word sums in HBM go with register group sums, in single runs"
 #endif
 
 // A node record decoded into registers (either table layout).
@@ -322,8 +328,8 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
 #define LSTRIDE MBG_BLOCK
 #endif
 #define LANE_X(w) lane[(0 * MBG_NW + (w)) * LSTRIDE + tid]
-#if MBG_STATE_HBM == 2
-// act alone in HBM, word w of grid thread atid at w * lane_stride + atid
+#if MBG_STATE_HBM >= 2
+// act in HBM, word w of grid thread atid at w * lane_stride + atid
 #define LANE_A(w) D.lane_state[(u64)(w) * D.lane_stride + atid]
 #else
 #define LANE_A(w) lane[(1 * MBG_NW + (w)) * LSTRIDE + tid]
@@ -379,8 +385,13 @@ __device__ __forceinline__ u32 eval_prog(const mbg_desc& D, const char* tb, u32 
     } while (0)
 #endif
 #if MBG_WSUM_REG == 2
+#if MBG_STATE_HBM == 3
+// the word sums themselves in HBM after act, word w of grid thread atid at (NW + w) * lane_stride + atid
+#define LANE_W(w) D.lane_state[(u64)(MBG_NW + (w)) * D.lane_stride + atid]
+#else
 // the word sums themselves in LDS (lane-interleaved like x and act)
 #define LANE_W(w) lane[(2 * MBG_NW + (w)) * LSTRIDE + tid]
+#endif
 #define WS_ADD(w, d)                                                       \
     do {                                                                   \
         const u32 w2_ = (w);                                               \
@@ -899,7 +910,7 @@ __device__ __forceinline__ void err_window(const mbg_desc& D, u64* thl, u32 t, u
 __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* tb, u64* lane, u64* thl) {
     // index of this lane's state: its thread in the block (LDS) or in the grid (HBM)
     const u32 tid = MBG_STATE_HBM == 1 ? blockIdx.x * MBG_BLOCK + threadIdx.x : threadIdx.x;
-#if MBG_STATE_HBM == 2
+#if MBG_STATE_HBM >= 2
     const u32 atid = blockIdx.x * MBG_BLOCK + threadIdx.x;  // this lane's column of act in HBM
 #endif
     const u32 lid = threadIdx.x & 63u;
@@ -1041,7 +1052,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #endif
             for (u32 q0 = 0; q0 < nq; q0 += DEP_BATCH) {
                 u32 kk[DEP_BATCH], xo[DEP_BATCH], xn[DEP_BATCH];
-#if MBG_STATE_HBM == 2
+#if MBG_STATE_HBM >= 2
                 u64 aw[DEP_BATCH];
 #endif
                 u64 rn[DEP_BATCH], rp[DEP_BATCH];
@@ -1052,7 +1063,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     const u32 dp = TDEP(pend_off + (ok ? q0 + u : 0u));
                     kk[u] = dp & 0xffffffu;
                     const u32 pos = dp >> 24;
-#if MBG_STATE_HBM == 2
+#if MBG_STATE_HBM >= 2
                     // the act word is read from HBM before the key gather, whose LDS reads
                     // and logic then cover its latency
                     aw[u] = LANE_A(kk[u] >> 6);
@@ -1070,7 +1081,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     {
                         // both table reads unconditional (valid entries), then masked
                         const u32 logic = L.logic_of(nk[u], key);
-#if MBG_STATE_HBM == 2
+#if MBG_STATE_HBM >= 2
                         const u32 a_old = (u32)(aw[u] >> (kk[u] & 63)) & 1u;
 #else
                         const u32 a_old = (u32)(LANE_A(kk[u] >> 6) >> (kk[u] & 63)) & 1u;
@@ -1088,7 +1099,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                         const u64 bit = 1ull << (kk[u] & 63);
                         WS_ADD(wk, rn[u] - rp[u]);
                         total += rn[u] - rp[u];
-#if MBG_STATE_HBM == 2
+#if MBG_STATE_HBM >= 2
                         // act words of two dependents in one batch may coincide: the batch's
                         // earlier writes are in memory, so read the word again only then
                         u64 a = aw[u];
@@ -1362,7 +1373,7 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
             WS_ADD(wj, 0ull - r_sel);
             total -= r_sel;
             // the hit's act word was read by the search (HBM act: no second read)
-            LANE_A(wj) = (MBG_STATE_HBM == 2 && hit ? a_hit : LANE_A(wj)) & ~(1ull << (j & 63));
+            LANE_A(wj) = (MBG_STATE_HBM >= 2 && hit ? a_hit : LANE_A(wj)) & ~(1ull << (j & 63));
             if (nj.flags & F_RO) {
                 ro_R -= r_sel;
                 ro_S -= L.term(nj, (nj.flags & F_RO) - 1u, xj_old, r_sel);
@@ -1404,7 +1415,7 @@ extern "C" __global__ __launch_bounds__(MBG_BLOCK) void mbg_simulate(mbg_desc D)
     u64* part = after;
 #else
     u64* lane = after;
-    u64* part = lane + (MBG_STATE_HBM == 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
+    u64* part = lane + (MBG_STATE_HBM >= 2 ? 1 : MBG_WSUM_REG == 1 ? 2 : 3) * MBG_NW * MBG_BLOCK;
 #endif
 #if MBG_TH_LDS
     // the block's partial sums follow the lanes' state (or the tables): th[W], then

@@ -414,10 +414,16 @@ const Module& get_module(mbg_ctx* c, const mbg::CompiledModel& cm, const Shape& 
 
 
 // LDS of the lanes' state: x and act, plus the word rate sums unless they are in VGPRs only;
-// x alone with act in HBM (state_hbm 2), nothing with the whole state in HBM (state_hbm 1)
+// x alone with act (state_hbm 2) or act and the word sums (3) in HBM, nothing with the whole
+// state in HBM (state_hbm 1)
 size_t lane_lds_bytes(int nw, int block, int wsum_reg, int state_hbm = 0) {
     if (state_hbm == 1) return 0;
-    return (size_t)(state_hbm == 2 ? 1 : wsum_reg == 1 ? 2 : 3) * nw * block * 8;
+    return (size_t)(state_hbm >= 2 ? 1 : wsum_reg == 1 ? 2 : 3) * nw * block * 8;
+}
+
+// 64-bit words per lane of the HBM lane-state buffer
+int lane_hbm_words(int nw, int wsum_reg, int state_hbm) {
+    return (state_hbm == 2 ? 1 : state_hbm == 3 ? 2 : wsum_reg == 1 ? 2 : 3) * nw;
 }
 
 int env_int(const char* name, int dflt) {
@@ -810,7 +816,7 @@ Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::R
         act_hbm_block(c, cm.NW, (lds.tables_lds ? blob_bytes : 0) + th_bytes, 0, &waves2);
         if (waves2 > waves0) hbm_mode = 2;
     }
-    const bool hbm = hbm_mode == 1;
+    const bool hbm = hbm_mode == 1 || hbm_mode == 3;
     // (act alone in HBM sets its own block below: the LDS-layout pass only picks the rest)
     const bool act_hbm = hbm_mode == 2 && act_ok;
     Shape sh = choose_shape(c, cm, blob_bytes, act_hbm ? 64 : block_req, trace, rate32 ? 1 : 0, !hbm);
@@ -819,6 +825,18 @@ Shape default_shape(const mbg_ctx* c, const mbg::CompiledModel& cm, const mbg::R
     sh.sparse = cm.K > mbg::kMaxReadouts ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
     if (hbm) make_state_hbm(sh, blob_bytes, block_req);
+    if (hbm_mode == 3 && sh.wsum_reg == 2) {
+        // x alone in LDS, act and the word sums in HBM (asked for; the autotuner measures it)
+        make_state_hbm(sh, blob_bytes, 0);
+        sh.state_hbm = 3;
+        sh.tables_lds = 0;
+        const int breq = block_req ? block_req : env_int("MBG_TUNE_BLOCK", 256);
+        const size_t th_bytes = (size_t)std::ceil(cm.max_time / cm.time_tick) * 8;
+        int block = 0;
+        for (int b = breq; b >= 64 && !block; b -= 64)
+            if (lane_lds_bytes(cm.NW, b, 2, 3) + th_bytes <= (size_t)c->lds_per_block) block = b;
+        sh.block = block ? block : 64;
+    }
     if (act_hbm && act_hbm_allowed(sh.wsum_reg, sh.sumw)) {
         // act alone in HBM: x's LDS sets the block (a requested one if it fits), tables in LDS
         // where they fit beside it
@@ -954,14 +972,14 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     // 79-word lanes ran 307 ms at 65,536 lanes (124 MB) and 459 ms at 262,144 (497 MB).
     uint64_t grid_max = p->grid ? (uint64_t)p->grid : (uint64_t)c->cus * (uint64_t)blocks_per_cu;
     if (sh.state_hbm && !p->grid) {
-        const uint64_t lane_bytes = (uint64_t)(sh.state_hbm == 2 ? 1 : sh.wsum_reg == 1 ? 2 : 3) * (uint64_t)cm.NW * 8;
+        const uint64_t lane_bytes = (uint64_t)lane_hbm_words(cm.NW, sh.wsum_reg, sh.state_hbm) * 8;
         const uint64_t fit = (uint64_t)env_int("MBG_TUNE_STATE_MB", 128) * (1ull << 20) / (lane_bytes * (uint64_t)sh.block);
         grid_max = std::max<uint64_t>((uint64_t)c->cus, std::min<uint64_t>(grid_max, fit));
     }
     if (sh.state_hbm) {
         const uint64_t stride = grid_max * (uint64_t)sh.block;
         if (stride > 0xffffffffull / (3ull * (uint64_t)cm.NW)) throw Error(MBG_ERR_LIMIT, "grid too large for the lanes' state");
-        B.lane_state.alloc((size_t)stride * (sh.state_hbm == 2 ? 1 : sh.wsum_reg == 1 ? 2 : 3) * cm.NW);
+        B.lane_state.alloc((size_t)stride * lane_hbm_words(cm.NW, sh.wsum_reg, sh.state_hbm));
         D.lane_state = B.lane_state.p;
         D.lane_stride = (uint32_t)stride;
     }
@@ -1230,6 +1248,25 @@ void autotune_impl(mbg_ctx* c, const mbg_model* m, uint64_t sample, int trace, m
         Shape h = base;
         make_state_hbm(h, pack_blob(cm, rt, r32, base.compact != 0).lay.bytes, 0);
         consider(h);
+        // x alone in LDS, act and the word sums in HBM (two-level sums, tables in HBM): the
+        // dependents' input gathers and the flips read LDS instead of the HBM buffer; the
+        // largest block whose x fits beside the window partial sums, and blocks of 256
+        if (base.wsum_reg == 2) {
+            const size_t th_bytes = (size_t)std::ceil(cm.max_time / cm.time_tick) * 8;
+            int prev = 0;
+            for (int cap : {1024, 256}) {
+                int block = 0;
+                for (int b = cap; b >= 64 && !block; b -= 64)
+                    if (lane_lds_bytes(cm.NW, b, 2, 3) + th_bytes <= (size_t)c->lds_per_block) block = b;
+                if (!block || block == prev) continue;
+                prev = block;
+                Shape x = h;
+                x.state_hbm = 3;
+                x.tables_lds = 0;
+                x.block = block;
+                consider(x);
+            }
+        }
         // act alone in HBM (x in LDS, word sums in VGPRs): blocks of at most 12 waves (3 per
         // SIMD leave the step loop its ~170 VGPRs without spills), and the fullest block
         if (act_hbm_allowed(base.wsum_reg, base.sumw)) {
