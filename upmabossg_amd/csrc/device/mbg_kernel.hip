@@ -1274,25 +1274,27 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
         // sgs - rem - 1 from the top (hit when rate >= sgs - rem), and from the top the active
         // bits are read bit-reversed so that the lowest set bit is always the next candidate.
 #if MBG_WSUM_REG == 2
-        // second level: the word sums of the selected group, in LDS
+        // second level: the word sums of the selected group (in LDS, or in HBM), all read
+        // before the first is used so that the reads overlap, then selected branch-free
         u32 w0 = (u32)(gsel * MBG_SUMW);
         {
             bool f2 = false;
             u64 sw_sel = 1;
             u32 wsel = w0;
+            u64 swq[MBG_SUMW];
 #pragma unroll
             for (int q = 0; q < MBG_SUMW; q++) {
                 const u32 w = (u32)(gsel * MBG_SUMW + q);
-                if (w < (u32)MBG_NW && !f2) {
-                    const u64 sw = LANE_W(w);
-                    if (acc + sw > target) {
-                        wsel = w;
-                        sw_sel = sw;
-                        f2 = true;
-                    } else {
-                        acc += sw;
-                    }
-                }
+                swq[q] = w < (u32)MBG_NW ? LANE_W(w) : 0ull;
+            }
+#pragma unroll
+            for (int q = 0; q < MBG_SUMW; q++) {
+                const u32 w = (u32)(gsel * MBG_SUMW + q);
+                const bool h2 = !f2 && w < (u32)MBG_NW && acc + swq[q] > target;
+                wsel = h2 ? w : wsel;
+                sw_sel = h2 ? swq[q] : sw_sel;
+                acc = (f2 || h2 || w >= (u32)MBG_NW) ? acc : acc + swq[q];
+                f2 = f2 || h2;
             }
             found = found && f2;
             sgs = sw_sel;
