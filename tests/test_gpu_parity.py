@@ -427,6 +427,27 @@ def test_large_full_size_statistics_independent_seed(gpu_ctx):
     assert np.all(np.abs(zf) <= _stat_report("final states", zf))
 
 
+def test_fixed_point_dedup_with_equal_leading_words(gpu_ctx, monkeypatch):
+    """Fixed points that agree on their two most significant words (nodes 128-199 frozen OFF
+    in a 200-node network, so those words are 0 in every state): the device deduplication
+    orders them by the lower words exactly as sorting every record on the host does."""
+    net = synth.random_network(200, 1, 3, 21, rates="loguniform", istate="uniform", readouts=10,
+                               special_frac=0.3, max_time=200.0, time_tick=2.0, sample_count=20000)
+    net.up[128:] = 0.0
+    net.down[128:] = 0.0
+    net.istate[128:] = 0.0
+    m = Model(net.bnd(), net.cfg())
+    _flush("case: 200 nodes, frozen leading words, device deduplication")
+    a = gpu_ctx.run(m, seed=3)
+    assert a.fixed_point_trajectories > 4096 and len(a.fixed_counts) > 100
+    assert not a.fixed_states[:, 2:].any()
+    _flush("case: 200 nodes, frozen leading words, host deduplication")
+    monkeypatch.setenv("MBG_TUNE_FP_HOST", "1")
+    h = gpu_ctx.run(m, seed=3)
+    monkeypatch.delenv("MBG_TUNE_FP_HOST")
+    assert np.array_equal(a.fixed_states, h.fixed_states) and np.array_equal(a.fixed_counts, h.fixed_counts)
+
+
 def test_fixed_point_heavy_full_size_statistics_and_dedup(gpu_ctx, monkeypatch):
     """262,144 GPU trajectories of the 5,000-node fixed-point-heavy network (seed 1) against an
     independent oracle sample (384 trajectories, seed 2): the fraction reaching a fixed point,

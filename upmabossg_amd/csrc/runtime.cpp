@@ -335,6 +335,7 @@ struct mbg_ctx {
     // fixed-point deduplication on the device
     DevBuf<uint32_t> fp_keys, fp_idx;
     DevBuf<uint64_t> fp_counts, fp_cnt, fp_n, fp_out;
+    mbg::PinnedU64 fp_host, fp_host_cnt;  // distinct fixed points read back (kept: no refaults)
     std::vector<hipStream_t> class_streams;  // one per word class of a sweep
     // last run
     bool have = false;
@@ -627,9 +628,11 @@ void unique_fixed_points(const std::vector<uint64_t>& fps, int words, int keep, 
 // (mbg_fp_dedup / _compact / _gather), so only those cross PCIe.
 void collect_fixed_points(mbg_ctx* c, const Module& mod, const uint64_t* d_fp, uint64_t nfp, int stride, int keep,
                           hipStream_t s, std::vector<uint64_t>& states, std::vector<uint64_t>& counts) {
-    states.clear();
-    counts.clear();
-    if (nfp == 0) return;
+    if (nfp == 0) {
+        states.clear();
+        counts.clear();
+        return;
+    }
     // MBG_TUNE_FP_HOST=1: every record to the host and sorted there (the check of the device path)
     if (nfp <= 4096 || env_int("MBG_TUNE_FP_HOST", 0) != 0) {
         std::vector<uint64_t> fps((size_t)nfp * stride);
@@ -667,25 +670,60 @@ void collect_fixed_points(mbg_ctx* c, const Module& mod, const uint64_t* d_fp, u
     void* a3[] = {(void*)&fp, &st, &kw, &oidx, &m, &out};
     hip_check(hipModuleLaunchKernel(mod.fp_gather, (unsigned)((m * keep + 255) / 256), 1, 1, 256, 1, 1, 0, s, a3, nullptr),
               "launch mbg_fp_gather");
-    std::vector<uint64_t> dist((size_t)m * keep), cnt(m);
-    hip_check(hipMemcpyAsync(dist.data(), c->fp_out.p, dist.size() * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-    hip_check(hipMemcpyAsync(cnt.data(), c->fp_cnt.p, m * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    // into pinned buffers kept by the context (a full-speed copy, and no page faults or zero
+    // fill on later runs)
+    if (c->fp_host.size() < (size_t)m * keep) c->fp_host.resize((size_t)m * keep);
+    if (c->fp_host_cnt.size() < m) c->fp_host_cnt.resize(m);
+    const uint64_t* dist = c->fp_host.data();
+    const uint64_t* cnt = c->fp_host_cnt.data();
+    hip_check(hipMemcpyAsync(c->fp_host.data(), c->fp_out.p, (size_t)m * keep * 8, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(c->fp_host_cnt.data(), c->fp_cnt.p, m * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(s), "fixed-point gather");
-    std::vector<uint64_t> idx(m);
-    std::iota(idx.begin(), idx.end(), 0);
-    auto row = [&](uint64_t r) { return dist.data() + r * keep; };
-    std::sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) {
-        const uint64_t *x = row(a), *y = row(b);
-        for (int w = keep - 1; w >= 0; w--)
-            if (x[w] != y[w]) return x[w] < y[w];
+    // ordered by a key of the two most significant words, the whole rows deciding ties; large
+    // sets sorted in slices on host threads, then merged
+    struct Key {
+        uint64_t a, b, r;
+    };
+    auto row = [&](uint64_t r) { return dist + r * keep; };
+    std::vector<Key> ks(m);
+    for (uint64_t r = 0; r < m; r++) ks[r] = {row(r)[keep - 1], keep > 1 ? row(r)[keep - 2] : 0ull, r};
+    auto less = [&](const Key& x, const Key& y) {
+        if (x.a != y.a) return x.a < y.a;
+        if (x.b != y.b) return x.b < y.b;
+        const uint64_t *p = row(x.r), *q = row(y.r);
+        for (int w = keep - 3; w >= 0; w--)
+            if (p[w] != q[w]) return p[w] < q[w];
         return false;
-    });
-    states.reserve((size_t)m * keep);
-    counts.reserve(m);
-    for (uint64_t r : idx) {
-        states.insert(states.end(), row(r), row(r) + keep);
-        counts.push_back(cnt[r]);
+    };
+    const int nt = m >= 65536 ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
+    std::vector<size_t> off(nt + 1);
+    for (int t = 0; t <= nt; t++) off[t] = (size_t)(m * (uint64_t)t / (uint64_t)nt);
+    auto on_threads = [&](auto&& f) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(f, t);
+        f(0);
+        for (auto& x : th) x.join();
+    };
+    on_threads([&](int t) { std::sort(ks.begin() + (long)off[t], ks.begin() + (long)off[t + 1], less); });
+    for (int w = 1; w < nt; w *= 2) {
+        std::vector<std::thread> th;
+        for (int t = 0; t + w < nt; t += 2 * w)
+            th.emplace_back([&, t, w] {
+                std::inplace_merge(ks.begin() + (long)off[t], ks.begin() + (long)off[t + w],
+                                   ks.begin() + (long)off[std::min(t + 2 * w, nt)], less);
+            });
+        for (auto& x : th) x.join();
     }
+    // every element is written below: resizing a vector that held as many before fills nothing
+    states.resize((size_t)m * keep);
+    counts.resize(m);
+    on_threads([&](int t) {
+        for (size_t k = off[t]; k < off[t + 1]; k++) {
+            std::memcpy(states.data() + k * keep, row(ks[k].r), (size_t)keep * 8);
+            counts[k] = cnt[ks[k].r];
+        }
+    });
 }
 
 // Structure of a model that decides which launch shapes are possible and how they perform.
