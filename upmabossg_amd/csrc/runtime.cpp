@@ -1109,6 +1109,8 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
             if (cm.NW > 128) launch(mod.init_readouts, (D.batch + 255) / 256, 256, 0, s, D, "launch mbg_init_readouts");
             uint64_t grid = grid_max;
             grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (D.batch + sh.block - 1) / sh.block));
+            if (sh.state_hbm && grid * (uint64_t)sh.block > (uint64_t)D.lane_stride)
+                throw Error(MBG_ERR_ARG, "internal error: the grid's lanes do not fit the lane-state buffer");
             hip_check(hipEventRecord(e0, s), "hipEventRecord");
             launch(mod.sim, grid, sh.block, smem, s, D, "launch mbg_simulate");
             hip_check(hipEventRecord(e1, s), "hipEventRecord");
@@ -1709,6 +1711,12 @@ void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int
                 // one column per thread of the class's largest grid (cus x blocks_per_cu blocks)
                 D.lane_state = C.act.p;
                 D.lane_stride = (uint32_t)((uint64_t)c->cus * (uint64_t)C.blocks_per_cu * (uint64_t)C.sh.block);
+                // element (array, word) of grid thread t at (array * NW + word) * lane_stride + t:
+                // every launched thread needs its own column inside the allocation
+                if (C.grid * (uint64_t)C.sh.block > (uint64_t)D.lane_stride ||
+                    (uint64_t)D.lane_stride * (uint64_t)C.sh.nw * (C.sh.state_hbm == 2 ? 1u : C.sh.wsum_reg == 1 ? 2u : 3u) >
+                        (uint64_t)C.act.cap)
+                    throw Error(MBG_ERR_ARG, "internal error: a sweep class's lanes do not fit its lane-state buffer");
             }
             if (errors) {
                 D.prob2 = B.prob2.p;
