@@ -258,6 +258,23 @@ def test_x_in_lds_active_sets_and_sums_in_hbm_bit_exact(gpu_ctx, monkeypatch, ca
     assert gpu_ctx.info().state_hbm == 3
 
 
+@pytest.mark.parametrize("case", ["random", "large"])
+def test_generated_logic_bit_exact(gpu_ctx, monkeypatch, case):
+    """The dependents' logic from code generated per node (MBG_TUNE_GENCODE, a switch on the node
+    id with each node's logic expression written out) instead of the truth tables: the same
+    results as the oracle."""
+    monkeypatch.setenv("MBG_TUNE_GENCODE", "1")
+    if case == "random":
+        net = synth.random_network(300, 1, 6, 7, rates="loguniform", istate="uniform", readouts=12,
+                                   special_frac=0.2, max_time=20.0, time_tick=0.5, sample_count=3000)
+        count = 3000
+    else:
+        net = synth.config_large()
+        count = 2048
+    assert "#define MBG_GENCODE 1" in Model(net.bnd(), net.cfg()).generate_source()
+    _compare(gpu_ctx, net.bnd(), net.cfg(), count=count, n=net.sample_count)
+
+
 def test_more_than_128_words_bit_exact(gpu_ctx):
     """8,300 nodes (130 words): the initial states are drawn per (trajectory, word), the readout
     part of each staged state by its own pass, and the lanes' state (3,120 bytes per lane, more

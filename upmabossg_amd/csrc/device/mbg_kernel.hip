@@ -26,6 +26,8 @@
 //   MBG_ERRORS      1: also the sums of squares behind the ErrorProba / ErrorTH columns (each
 //                   trajectory's time per projected state and TH x time over each window,
 //                   squared); each lane keeps its current window's (state, time) list in HBM
+//   MBG_GENCODE     1: the dependents' logic from code generated per node (appended to this
+//                   source) instead of the truth tables (a comparison; the tables are faster)
 //   MBG_SWEEP_XCD   sweep waves start at variant 0, the costliest per trajectory (2), at variants
 //                   grouped by block and XCD (1), or spread per wave (0)
 //   MBG_STATE_HBM   1: the lanes' x, act and word sums in an HBM buffer (lane-interleaved over
@@ -148,6 +150,9 @@ typedef unsigned int sidx;
 #endif
 #ifndef MBG_SWEEP_XCD
 #define MBG_SWEEP_XCD 0
+#endif
+#ifndef MBG_GENCODE
+#define MBG_GENCODE 0
 #endif
 // The active sets alone in HBM keep x alone in LDS: the rate sums must be in registers.
 #if MBG_STATE_HBM == 2 && MBG_WSUM_REG != 1
@@ -758,6 +763,12 @@ struct Lane {
     }
 };
 
+#if MBG_GENCODE
+// Logic value of fast node i evaluated by code generated for the model (a switch on the node
+// id, appended to this source), instead of its truth table
+__device__ u32 mbg_gen_logic(u32 i, const Lane& L);
+#endif
+
 #if MBG_SPARSE
 #define MBG_EMPTY_CELL (~0ull)
 #define MBG_CELL_PROBES 64
@@ -1069,7 +1080,11 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
                     aw[u] = LANE_A(kk[u] >> 6);
 #endif
                     nk[u] = TNODE(kk[u]);
+#if MBG_GENCODE
+                    const u32 key = L.xbit(kk[u]);  // own state: the generated logic reads the inputs
+#else
                     const u32 key = L.key_of(nk[u], kk[u]);
+#endif
                     xn[u] = key & 1u;
                     xo[u] = pos == 0 ? xn[u] ^ 1u : xn[u];
 #if MBG_GENERIC
@@ -1080,7 +1095,11 @@ __device__ __forceinline__ void simulate_variant(const mbg_desc& D, const char* 
 #endif
                     {
                         // both table reads unconditional (valid entries), then masked
+#if MBG_GENCODE
+                        const u32 logic = mbg_gen_logic(kk[u], L);
+#else
                         const u32 logic = L.logic_of(nk[u], key);
+#endif
 #if MBG_STATE_HBM >= 2
                         const u32 a_old = (u32)(aw[u] >> (kk[u] & 63)) & 1u;
 #else

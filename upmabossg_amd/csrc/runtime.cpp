@@ -164,14 +164,111 @@ struct Shape {
     int nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch = 1, scan = 4, wsum_reg = 0, rate32 = 0,
         multi = 0, sumw = 1, compact = 0, wsmad = 0, sparse = 0, prog = 0, th_lds = 0, cells_lds = 0, dep_cap = 0,
         state_hbm = 0, errors = 0, sweep_xcd = 0;
+    // MBG_GENCODE: the fast nodes' logic as code generated for this model (appended to the
+    // kernel source; empty: the truth tables)
+    std::string gen;
     bool operator<(const Shape& o) const {
         return std::tie(nw, block, discrete, trace, generic, tables_lds, max_in, dep_batch, scan, wsum_reg, rate32, multi,
-                        sumw, compact, wsmad, sparse, prog, th_lds, cells_lds, dep_cap, state_hbm, errors, sweep_xcd) <
+                        sumw, compact, wsmad, sparse, prog, th_lds, cells_lds, dep_cap, state_hbm, errors, sweep_xcd,
+                        gen) <
                std::tie(o.nw, o.block, o.discrete, o.trace, o.generic, o.tables_lds, o.max_in, o.dep_batch, o.scan,
                         o.wsum_reg, o.rate32, o.multi, o.sumw, o.compact, o.wsmad, o.sparse, o.prog, o.th_lds,
-                        o.cells_lds, o.dep_cap, o.state_hbm, o.errors, o.sweep_xcd);
+                        o.cells_lds, o.dep_cap, o.state_hbm, o.errors, o.sweep_xcd, o.gen);
     }
 };
+
+// The logic of every node as generated code, the reference's way of compiling a model
+// (src/generator.cpp:84-122): one case per node in a switch on the node id, its `logic`
+// expression written out in the FP32 arithmetic the truth tables are built with
+// (model.cpp, StateEval).  Empty when some node has no plain logic attribute (generic rates,
+// logic programs) or an alias recursion.
+struct GenLogic {
+    const mbg::ParsedModel& pm;
+    int self = 0, depth = 0;
+    bool ok = true;
+
+    static std::string lit(float v) {
+        char b[64];
+        std::snprintf(b, sizeof b, "%.9gf", (double)v);
+        std::string t(b);
+        if (t == "inff") return "__builtin_inff()";
+        if (t == "-inff") return "(-__builtin_inff())";
+        if (t == "nanf" || t == "-nanf") return "__builtin_nanf(\"\")";
+        return "(" + t + ")";
+    }
+    std::string emit(const mbg::Ast* e) {
+        using mbg::Ast;
+        if (!ok) return "0.f";
+        switch (e->kind) {
+            case Ast::Number: return lit(e->number);
+            case Ast::Variable: return lit(mbg::evaluate(e, pm));
+            case Ast::Group: return emit(e->kids[0].get());
+            case Ast::Node: {
+                const int i = pm.index_of(e->text);
+                if (i < 0) ok = false;
+                return "(float)L.xbit(" + std::to_string(i < 0 ? 0 : i) + "u)";
+            }
+            case Ast::Alias: {
+                const mbg::Ast* a = pm.nodes[self].attr(e->text.substr(1));
+                if (!a || ++depth > 64) {
+                    ok = false;
+                    return "0.f";
+                }
+                const std::string v = emit(a);
+                depth--;
+                return v;
+            }
+            case Ast::Unary: {
+                const std::string v = emit(e->kids[0].get());
+                if (e->oper == Ast::Sub) return "(-" + v + ")";
+                if (e->oper == Ast::Not) return "(" + v + " == 0.f ? 1.f : 0.f)";
+                return v;
+            }
+            case Ast::Ternary:
+                return "(" + emit(e->kids[0].get()) + " != 0.f ? " + emit(e->kids[1].get()) + " : " +
+                       emit(e->kids[2].get()) + ")";
+            case Ast::Binary: {
+                const std::string l = emit(e->kids[0].get()), r = emit(e->kids[1].get());
+                switch (e->oper) {
+                    case Ast::Add: return "(" + l + " + " + r + ")";
+                    case Ast::Sub: return "(" + l + " - " + r + ")";
+                    case Ast::Mul: return "(" + l + " * " + r + ")";
+                    case Ast::Div: return "(" + l + " / " + r + ")";
+                    case Ast::And: return "((" + l + " != 0.f && " + r + " != 0.f) ? 1.f : 0.f)";
+                    case Ast::Or: return "((" + l + " != 0.f || " + r + " != 0.f) ? 1.f : 0.f)";
+                    case Ast::Xor: return "(((" + l + " != 0.f) != (" + r + " != 0.f)) ? 1.f : 0.f)";
+                    case Ast::Eq: return "(" + l + " == " + r + " ? 1.f : 0.f)";
+                    case Ast::Ne: return "(" + l + " != " + r + " ? 1.f : 0.f)";
+                    case Ast::Lt: return "(" + l + " < " + r + " ? 1.f : 0.f)";
+                    case Ast::Le: return "(" + l + " <= " + r + " ? 1.f : 0.f)";
+                    case Ast::Gt: return "(" + l + " > " + r + " ? 1.f : 0.f)";
+                    case Ast::Ge: return "(" + l + " >= " + r + " ? 1.f : 0.f)";
+                    default: ok = false; return "0.f";
+                }
+            }
+        }
+        ok = false;
+        return "0.f";
+    }
+};
+
+std::string generate_logic_source(const mbg::ParsedModel& pm, const mbg::CompiledModel& cm) {
+    if (cm.any_generic || cm.any_prog) return std::string();
+    GenLogic g{pm};
+    std::ostringstream os;
+    os << "__device__ u32 mbg_gen_logic(u32 i, const Lane& L) {\n    switch (i) {\n";
+    for (int i = 0; i < cm.N; i++) {
+        const mbg::Ast* e = pm.nodes[i].attr("logic");
+        if (!e) return std::string();
+        g.self = i;
+        g.depth = 0;
+        const std::string v = g.emit(e);
+        if (!g.ok) return std::string();
+        os << "    case " << i << "u: return " << v << " != 0.f ? 1u : 0u;\n";
+    }
+    os << "    default: return 0u;\n    }\n}\n";
+    return os.str();
+}
 
 std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     std::ostringstream os;
@@ -200,8 +297,9 @@ std::string generate_source(const mbg::CompiledModel& cm, const Shape& sh) {
     os << "#define MBG_STATE_HBM " << sh.state_hbm << "\n";
     os << "#define MBG_ERRORS " << sh.errors << "\n";
     os << "#define MBG_SWEEP_XCD " << sh.sweep_xcd << "\n";
+    os << "#define MBG_GENCODE " << (sh.gen.empty() ? 0 : 1) << "\n";
     os << "#define MBG_HD __device__ __forceinline__\n";
-    os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n";
+    os << mbg_embedded_math << "\n" << mbg_embedded_kernel << "\n" << sh.gen;
     return os.str();
 }
 
@@ -928,6 +1026,11 @@ void run_impl(mbg_ctx* c, const mbg_model* m, const mbg_run_params* p, const Sha
     }
     sh.sparse = sparse ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
+    // MBG_TUNE_GENCODE=1: the fast nodes' logic as code generated for the model instead of
+    // truth tables (a per-lane switch on the node id; measured several times slower, kept for
+    // that comparison)
+    sh.gen.clear();
+    if (env_int("MBG_TUNE_GENCODE", 0) != 0 && m->parsed) sh.gen = generate_logic_source(*m->parsed, cm);
     const bool errors = p->error_columns != 0;
     sh.errors = errors ? 1 : 0;
     const Blob blob = pack_blob(cm, rt, sh.rate32 != 0, sh.compact != 0);
@@ -2300,6 +2403,7 @@ mbg_status mbg_model_generate_source(const mbg_model* m, int trace, char* buf, s
     Shape sh{cm.NW, 64, cm.discrete ? 1 : 0, trace ? 1 : 0, cm.any_generic ? 1 : 0, 0, std::max(1, cm.max_inputs)};
     sh.sparse = cm.K > mbg::kMaxReadouts ? 1 : 0;
     sh.prog = cm.any_prog ? 1 : 0;
+    if (env_int("MBG_TUNE_GENCODE", 0) != 0 && m->parsed) sh.gen = generate_logic_source(*m->parsed, cm);
     const std::string s = generate_source(cm, sh);
     if (len) *len = s.size();
     if (buf && cap) {
