@@ -77,6 +77,18 @@ def test_sparse_error_sums(gpu_ctx, readouts, discrete):
     _check(gpu_ctx, net.bnd(), cfg, count=3000)
 
 
+@pytest.mark.parametrize("state_hbm", ["1", "3"])
+def test_lane_state_in_hbm_error_sums(gpu_ctx, monkeypatch, state_hbm):
+    """The error sums with the lanes' state in HBM (1) or x alone in LDS with the active sets and
+    word sums in HBM (3), the layouts of wide models (each lane's window list is in HBM too)."""
+    monkeypatch.setenv("MBG_TUNE_STATE_HBM", state_hbm)
+    monkeypatch.setenv("MBG_TUNE_WSUM_REG", "2")
+    net = synth.random_network(300, 1, 6, 7, rates="loguniform", istate="uniform", readouts=12,
+                               special_frac=0.2, max_time=20.0, time_tick=0.5, sample_count=3000)
+    _check(gpu_ctx, net.bnd(), net.cfg(), count=3000)
+    assert gpu_ctx.info().state_hbm == int(state_hbm)
+
+
 def test_logic_programs_error_sums(gpu_ctx):
     b, c = synth.ring_network(100, steps_to_fixed_point=50, sample_count=2000, time_tick=5.0, max_time=100.0)
     _check(gpu_ctx, b, c, count=2000)
