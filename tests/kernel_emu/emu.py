@@ -153,7 +153,7 @@ def shape_for(variants, **over):
     with multi=1), with keyword overrides (nw, block, state_hbm, wsum_reg, sumw, ...)."""
     fs = facts(variants)
     nw = max(f["NW"] for f in fs)
-    db = over.get("dep_batch", 2)
+    db = over.get("dep_batch", 1)  # the runtime runs one dependent per round
     mean = max(f["mean_deps"] for f in fs)
     sh = dict(nw=nw, block=64, discrete=fs[0]["discrete"], trace=0, generic=int(any(f["generic"] for f in fs)),
               tables_lds=0, max_in=max(1, max(f["max_in"] for f in fs)), dep_batch=db, scan=4,

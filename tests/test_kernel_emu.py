@@ -80,11 +80,13 @@ def test_small_network_default_layout():
     _run([dict(bnd=net.bnd(), cfg=net.cfg(), n=400, seed=1)], block=64, tables_lds=1, wsum_reg=1)
 
 
-@pytest.mark.parametrize("layout", ["lds", "act_hbm", "state_hbm", "x_lds", "lds_sums"])
+@pytest.mark.parametrize("layout", ["lds", "act_hbm", "act_hbm_pairs", "state_hbm", "x_lds", "lds_sums"])
 def test_thousand_node_network_layouts(layout):
     v = _net(1000, count=64)
     over = dict(lds=dict(block=128, tables_lds=1, wsum_reg=1),
                 act_hbm=dict(block=128, tables_lds=1, wsum_reg=1, state_hbm=2),
+                # rounds of two dependents: kept in the source (the runtime refuses them)
+                act_hbm_pairs=dict(block=128, tables_lds=1, wsum_reg=1, state_hbm=2, dep_batch=2),
                 state_hbm=dict(block=64, wsum_reg=2, sumw=4, state_hbm=1),
                 x_lds=dict(block=128, wsum_reg=2, sumw=3, state_hbm=3),
                 lds_sums=dict(block=64, wsum_reg=0, dep_cap=0, scan=8))[layout]
