@@ -1494,8 +1494,11 @@ std::vector<uint64_t> variant_key(const mbg_variant& v) {
 void run_variants_impl(mbg_ctx* c, const mbg_variant* vs, int nv, int flags, int depth = 0) {
     const bool errors = (flags & MBG_SWEEP_ERROR_COLUMNS) != 0;
     const auto t0 = std::chrono::steady_clock::now();
-    if (c->sweep) {  // return the previous sweep's buffers to the pool
-        for (auto& V : c->sweep->vars) c->variant_pool.push_back(std::move(V));
+    if (c->sweep) {
+        // return the previous sweep's buffers to the pool, last variant first, so that a sweep of
+        // the same variants takes each one's buffers back at the sizes it needs (no reallocation)
+        for (auto it = c->sweep->vars.rbegin(); it != c->sweep->vars.rend(); ++it)
+            c->variant_pool.push_back(std::move(*it));
         c->sweep.reset();
     }
     auto sw = std::make_unique<SweepState>();
