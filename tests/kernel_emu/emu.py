@@ -61,12 +61,16 @@ def _host_objects(asan: bool, msan: bool = False):
         deps = [src] + [os.path.join(B.CSRC, h) for h in ("front.h", "model.h", "pack.h")]
         if os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
             continue
-        procs.append(subprocess.Popen([_cxx(msan), *_flags(asan, msan), "-c", src, "-o", obj], stdout=subprocess.PIPE,
-                                      stderr=subprocess.PIPE, text=True))
-    for p in procs:
+        # written under a name of this process, then renamed: test workers running in parallel
+        # may build the same object
+        tmp = f"{obj}.{os.getpid()}.tmp"
+        procs.append((tmp, obj, subprocess.Popen([_cxx(msan), *_flags(asan, msan), "-c", src, "-o", tmp],
+                                                 stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+    for tmp, obj, p in procs:
         o, e = p.communicate()
         if p.returncode:
             raise RuntimeError(f"emulator host build failed:\n{o}\n{e}")
+        os.replace(tmp, obj)
     return objs
 
 
@@ -87,14 +91,16 @@ def build(shape: dict, asan: bool = False, msan: bool = False) -> str:
     if os.path.exists(exe) and os.path.getmtime(exe) >= max(os.path.getmtime(o) for o in objs):
         return exe
     kfile = os.path.join(BUILD, f"k_{key}.h")
-    with open(kfile, "w") as f:
+    tmp = f"{exe}.{os.getpid()}.tmp"  # per process: parallel test workers may build the same shape
+    with open(f"{kfile}.{os.getpid()}.tmp", "w") as f:
         f.write(src)
+    os.replace(f"{kfile}.{os.getpid()}.tmp", kfile)
     cmd = [_cxx(msan), *_flags(asan, msan), f'-DEMU_KERNEL="k_{key}.h"', os.path.join(HERE, "emu_run.cpp"), *objs,
-           "-o", exe + ".tmp"]
+           "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode:
         raise RuntimeError(f"emulator build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(exe + ".tmp", exe)
+    os.replace(tmp, exe)
     return exe
 
 
@@ -103,10 +109,11 @@ def _planner() -> str:
     objs = _host_objects(False)
     src = os.path.join(HERE, "emu_plan.cpp")
     if not (os.path.exists(exe) and os.path.getmtime(exe) >= max(os.path.getmtime(x) for x in objs + [src])):
-        r = subprocess.run([CXX, *_flags(False), src, *objs, "-o", exe + ".tmp"], capture_output=True, text=True)
+        tmp = f"{exe}.{os.getpid()}.tmp"
+        r = subprocess.run([CXX, *_flags(False), src, *objs, "-o", tmp], capture_output=True, text=True)
         if r.returncode:
             raise RuntimeError(f"planner build failed:\n{r.stdout}\n{r.stderr}")
-        os.replace(exe + ".tmp", exe)
+        os.replace(tmp, exe)
     return exe
 
 
